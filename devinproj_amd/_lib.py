@@ -1,0 +1,157 @@
+"""ctypes binding of the HIP library (_accuhip.so) and device-buffer helpers.
+
+The library must be built (devinproj_amd/build_lib.py) and a ROCm GPU must be
+present: there is no CPU fallback, every entry point fails loudly otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_accuhip.so")
+
+ACCU_FP64 = 0
+ACCU_FP32_PAIR = 1
+PRECISIONS = {"fp64": ACCU_FP64, "mixed": ACCU_FP32_PAIR, "fp32": ACCU_FP32_PAIR}
+
+P = C.c_void_p
+I = C.c_int
+LL = C.c_longlong
+D = C.c_double
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "accu_last_error": [],
+    "accu_version": [],
+    "accu_device_info": [P, I, P, P],
+    "accu_sync": [P],
+    "accu_nlist_create": [P],
+    "accu_nlist_destroy": [P],
+    "accu_nlist_build": [P, P, I, P, P, P, P, I, D, P],
+    "accu_nlist_info": [P, P],
+    "accu_nlist_cells": [P, P, P, P, P],
+    "accu_nlist_count_half": [P, P, P],
+    "accu_nlist_export_half": [P, P, P, P],
+    "accu_max_disp2": [P, P, I, P, I, P, P],
+    "accu_lj_forces": [P, P, P, I, I, P, P, P, D, P, I, I, P, P, P],
+    "accu_coulomb_forces": [P, P, P, I, I, D, D, D, D, P, I, P, I, I, P, P, P],
+    "accu_ewald_recip": [P, P, I, P, D, P, I, D, I, P, P, P],
+    "accu_gravity_direct": [P, P, I, P, I, D, D, P, I, I, P, P, P, P],
+    "accu_bh_forces": [P, P, I, D, D, D, I, I, P, P, P, P],
+}
+_RESTYPES = {"accu_last_error": C.c_char_p}
+
+_lib = None
+_torch = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the library once (raises if it is not built or no GPU is visible)."""
+    global _lib, _torch
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built: run `python -m devinproj_amd.build_lib`")
+    import torch
+    if not torch.cuda.is_available():
+        raise LibraryMissing("devinproj_amd needs a ROCm GPU (gfx950); none is visible")
+    torch.cuda.init()
+    L = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _torch = torch
+    _lib = L
+    return L
+
+
+def load_symbols_only():
+    """Open the library without touching the GPU (for export checks on a CPU host)."""
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built")
+    return C.CDLL(LIB_PATH)
+
+
+def check(rc):
+    if rc == 0:
+        return
+    msg = _lib.accu_last_error().decode() if _lib is not None else "library error"
+    if rc == -1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))
+
+
+def torch():
+    lib()
+    return _torch
+
+
+def stream():
+    return P(_torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device (torch tensor) or host (numpy array) pointer; None -> NULL."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        assert t.flags.c_contiguous
+        return P(t.ctypes.data)
+    return P(t.data_ptr())
+
+
+def host(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Buffers:
+    """Per-object device buffers reused across calls while their shape is unchanged."""
+
+    def __init__(self):
+        self._b = {}
+
+    def get(self, name, shape, dtype):
+        th = torch()
+        t = self._b.get(name)
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype:
+            t = th.empty(shape, dtype=dtype, device="cuda")
+            self._b[name] = t
+        return t
+
+    def upload(self, name, arr, dtype):
+        th = torch()
+        a = np.ascontiguousarray(arr)
+        t = self.get(name, a.shape, dtype)
+        t.copy_(th.from_numpy(a))
+        return t
+
+
+def sync():
+    _torch.cuda.current_stream().synchronize()
+
+
+SIGNATURES.update({
+    "accu_md_create": [P, I, P, P, P, I, P, P, P, I],
+    "accu_md_destroy": [P],
+    "accu_md_set_lj": [P, I, P, P, P, D],
+    "accu_md_set_dsf": [P, D, D, D, D],
+    "accu_md_init": [P, D, D, I, D, I],
+    "accu_md_run": [P, I, D, P, I, I, I],
+    "accu_md_sync": [P],
+    "accu_md_get": [P, P, P, P, P, I, P],
+    "accu_md_stream": [P, P],
+    "accu_md_set_timing": [P, I],
+    "accu_md_timing": [P, P, P],
+})

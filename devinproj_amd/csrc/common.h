@@ -1,0 +1,170 @@
+// Shared device/host helpers for the accusim force-path library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#define ACCU_OK 0
+#define ACCU_EINVAL (-1)     // invalid physics setup / arguments  -> ValueError
+#define ACCU_ERUNTIME (-2)   // HIP / device failure                -> RuntimeError
+#define ACCU_EOVERFLOW (-3)  // a fixed-capacity buffer overflowed  -> RuntimeError
+
+namespace accu {
+void set_error(const char* fmt, ...);
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      accu::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return ACCU_ERUNTIME;                                                                  \
+    }                                                                                        \
+  } while (0)
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+#define ARG_CHECK(cond, ...)            \
+  do {                                  \
+    if (!(cond)) {                      \
+      accu::set_error(__VA_ARGS__);     \
+      return ACCU_EINVAL;               \
+    }                                   \
+  } while (0)
+
+namespace accu {
+
+constexpr int kXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// Map a dispatch-order block id to a logical block id so that each XCD works on one
+// contiguous range of atoms (its own L2 then holds one spatial window).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb / kXcd, rem = nb % kXcd;
+  const int x = b % kXcd, k = b / kXcd;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
+struct Box {
+  double L[3];
+  double inv[3];
+  int periodic;
+};
+
+// numpy's round-half-even minimum image d -= L * round(d / L) with the true
+// division (bit-for-bit the reference expression; no contraction).
+__device__ __forceinline__ double minimg_exact(double d, double L) {
+#pragma clang fp contract(off)
+  return d - L * rint(d / L);
+}
+
+// Same image choice with a reciprocal multiply: identical for every |d_min| < L/2
+// (only d/L within an ulp of a half-integer can round differently).
+__device__ __forceinline__ double minimg_fast(double d, double L, double inv) {
+  return fma(-L, rint(d * inv), d);
+}
+
+// |d|^2 summed as numpy's einsum("ij,ij->i") sums a 3-vector: (x^2 + z^2) + y^2.
+__device__ __forceinline__ double r2_np(double x, double y, double z) {
+#pragma clang fp contract(off)
+  return (x * x + z * z) + y * y;
+}
+__device__ __forceinline__ float r2_npf(float x, float y, float z) {
+#pragma clang fp contract(off)
+  return (x * x + z * z) + y * y;
+}
+
+// 1/x in float64 from v_rcp_f64 plus two Newton steps (within 1 ulp).
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/sqrt(x) in float64 from v_rsq_f64 plus two Newton steps.
+__device__ __forceinline__ double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Deterministic sum of `nv` values per thread over the block; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV * nwaves] */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * nw + w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < nw; ++q) s += lds[k * nw + q];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Grid-wide deterministic reduction: every block stores its NV partials, the last
+// block to arrive sums them in block order and writes out[0..NV). `counter` must be
+// zero on entry and is left zero. Blocks must be 1-D; gridDim.x partial slots.
+template <int NV>
+__device__ void grid_sum(double (&v)[NV], double* partials, unsigned* counter, double* out,
+                         double* lds) {
+  block_sum<NV>(v, lds);
+  __shared__ int am_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)blockIdx.x * NV + k] = v[k];
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  const int nb = gridDim.x;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  // fixed-order strided partial sums, then the fixed-order block sum
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<NV>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = acc[k];
+    *counter = 0u;
+  }
+}
+
+// Per-call reduction workspace owned by the library (one per entry-point family).
+struct Workspace {
+  double* partials = nullptr;
+  size_t cap = 0;  // doubles
+  unsigned* counter = nullptr;
+  double* out = nullptr;  // 16 doubles
+  int ensure(size_t ndoubles);
+};
+
+Workspace& default_workspace();
+
+}  // namespace accu

@@ -1,0 +1,65 @@
+// Library-wide state: error text, reduction workspace, version and device info.
+#include "common.h"
+#include "../../include/accusim_hip.h"
+
+#include <cstring>
+
+namespace accu {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+int Workspace::ensure(size_t ndoubles) {
+  if (!counter) {
+    HIP_TRY(hipMalloc(&counter, 64));
+    HIP_TRY(hipMemset(counter, 0, 64));
+    HIP_TRY(hipMalloc(&out, 16 * sizeof(double)));
+  }
+  if (ndoubles > cap) {
+    if (partials) HIP_TRY(hipFree(partials));
+    size_t c = ndoubles < 4096 ? 4096 : ndoubles;
+    HIP_TRY(hipMalloc(&partials, c * sizeof(double)));
+    cap = c;
+  }
+  return ACCU_OK;
+}
+
+Workspace& default_workspace() {
+  static Workspace ws;
+  return ws;
+}
+
+}  // namespace accu
+
+extern "C" {
+
+const char* accu_last_error(void) { return accu::g_err; }
+
+int accu_version(void) { return ACCU_ABI_VERSION; }
+
+int accu_device_info(char* name, int len, int* cus, long long* hbm_bytes) {
+  hipDeviceProp_t p;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  if (name && len > 0) {
+    strncpy(name, p.gcnArchName, len - 1);
+    name[len - 1] = 0;
+  }
+  if (cus) *cus = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (long long)p.totalGlobalMem;
+  return ACCU_OK;
+}
+
+int accu_sync(void* stream) {
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return ACCU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// O(N^2) softened Newtonian gravity, j-particles staged through LDS tiles.
+// forces.py:38-48: F_i = sum_j G m_i m_j (r_j - r_i) / (r^2 + eps^2)^(3/2),
+//                  U   = -1/2 sum_{i != j} G m_i m_j / sqrt(r^2 + eps^2).
+// tree.py:81-87 (direct_subset) is the same sum restricted to target rows.
+#include "common.h"
+#include "../../include/accusim_hip.h"
+
+namespace accu {
+
+constexpr int kGT = 256;  // threads per block = j-tile size
+
+Box make_box(const double* box, int periodic);
+
+// FP64: every pair term and sum in float64.
+__global__ void __launch_bounds__(kGT) k_grav64(const double* __restrict__ pos, const double* __restrict__ m, int n,
+                                                const long long* __restrict__ tgt, int ntgt, double G, double eps2,
+                                                Box box, double* __restrict__ F, double* partials,
+                                                unsigned* counter, double* Uout) {
+  __shared__ double4 tile[kGT];
+  __shared__ double red[16];
+  const int r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const bool active = r < ntgt;
+  const int i = active ? (tgt ? (int)tgt[r] : r) : 0;
+  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+  double fx = 0, fy = 0, fz = 0, phi = 0;
+  for (int base = 0; base < n; base += kGT) {
+    const int j = base + threadIdx.x;
+    if (j < n) tile[threadIdx.x] = make_double4(pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2], m[j]);
+    __syncthreads();
+    const int cnt = (n - base) < kGT ? (n - base) : kGT;
+    for (int k = 0; k < cnt; ++k) {
+      const double4 pj = tile[k];
+      double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+      if (box.periodic) {
+        dx = minimg_fast(dx, box.L[0], box.inv[0]);
+        dy = minimg_fast(dy, box.L[1], box.inv[1]);
+        dz = minimg_fast(dz, box.L[2], box.inv[2]);
+      }
+      const double r2 = r2_np(dx, dy, dz) + eps2;
+      double inv = rsqrt64(r2);
+      inv = (base + k == i) ? 0.0 : inv;
+      const double mi3 = pj.w * inv * inv * inv;
+      fx = fma(mi3, dx, fx);
+      fy = fma(mi3, dy, fy);
+      fz = fma(mi3, dz, fz);
+      phi = fma(pj.w, inv, phi);
+    }
+    __syncthreads();
+  }
+  double U[1] = {0.0};
+  if (active) {
+    const double gm = G * m[i];
+    F[3 * (size_t)r] = gm * fx;
+    F[3 * (size_t)r + 1] = gm * fy;
+    F[3 * (size_t)r + 2] = gm * fz;
+    U[0] = -0.5 * gm * phi;
+  }
+  if (Uout) grid_sum<1>(U, partials, counter, Uout, red);
+}
+
+// Mixed: float32 pair terms summed per tile in float32, tile sums added in float64.
+// Positions are taken relative to `origin` (the centre of mass) before rounding.
+__global__ void __launch_bounds__(kGT) k_grav32(const double* __restrict__ pos, const double* __restrict__ m, int n,
+                                                const long long* __restrict__ tgt, int ntgt, double G, double eps2,
+                                                double ox, double oy, double oz, double* __restrict__ F,
+                                                double* partials, unsigned* counter, double* Uout) {
+  __shared__ float4 tile[kGT];
+  __shared__ double red[16];
+  const int r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const bool active = r < ntgt;
+  const int i = active ? (tgt ? (int)tgt[r] : r) : 0;
+  const float xi = (float)(pos[3 * (size_t)i] - ox), yi = (float)(pos[3 * (size_t)i + 1] - oy),
+              zi = (float)(pos[3 * (size_t)i + 2] - oz);
+  const float e2 = (float)eps2;
+  double fx = 0, fy = 0, fz = 0, phi = 0;
+  for (int base = 0; base < n; base += kGT) {
+    const int j = base + threadIdx.x;
+    if (j < n)
+      tile[threadIdx.x] = make_float4((float)(pos[3 * (size_t)j] - ox), (float)(pos[3 * (size_t)j + 1] - oy),
+                                      (float)(pos[3 * (size_t)j + 2] - oz), (float)m[j]);
+    __syncthreads();
+    const int cnt = (n - base) < kGT ? (n - base) : kGT;
+    float gx = 0.f, gy = 0.f, gz = 0.f, gp = 0.f;
+    for (int k = 0; k < cnt; ++k) {
+      const float4 pj = tile[k];
+      const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
+      float inv = __builtin_amdgcn_rsqf(r2);
+      inv = (base + k == i) ? 0.f : inv;
+      const float mi = pj.w * inv;
+      const float mi3 = mi * inv * inv;
+      gx = fmaf(mi3, dx, gx);
+      gy = fmaf(mi3, dy, gy);
+      gz = fmaf(mi3, dz, gz);
+      gp += mi;
+    }
+    fx += (double)gx;
+    fy += (double)gy;
+    fz += (double)gz;
+    phi += (double)gp;
+    __syncthreads();
+  }
+  double U[1] = {0.0};
+  if (active) {
+    const double gm = G * m[i];
+    F[3 * (size_t)r] = gm * fx;
+    F[3 * (size_t)r + 1] = gm * fy;
+    F[3 * (size_t)r + 2] = gm * fz;
+    U[0] = -0.5 * gm * phi;
+  }
+  if (Uout) grid_sum<1>(U, partials, counter, Uout, red);
+}
+
+}  // namespace accu
+
+using namespace accu;
+
+extern "C" {
+
+int accu_gravity_direct(const double* pos, const double* mass, int n, const long long* targets, int ntgt, double G,
+                        double eps2, const double* box, int periodic, int precision, const double* origin,
+                        double* F, double* U, void* stream) {
+  ARG_CHECK(pos && mass && F, "null argument");
+  ARG_CHECK(precision == ACCU_FP64 || precision == ACCU_FP32_PAIR, "unknown precision %d", precision);
+  ARG_CHECK(!(precision == ACCU_FP32_PAIR && periodic), "float32 gravity supports open boundaries only");
+  if (!targets) ntgt = n;
+  if (n <= 0 || ntgt <= 0) return ACCU_OK;
+  hipStream_t st = (hipStream_t)stream;
+  Workspace& ws = default_workspace();
+  const int nb = (ntgt + kGT - 1) / kGT;
+  int rc = ws.ensure((size_t)nb * 2);
+  if (rc) return rc;
+  if (precision == ACCU_FP64) {
+    Box b = make_box(box, periodic);
+    hipLaunchKernelGGL(k_grav64, dim3(nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, b, F, ws.partials,
+                       ws.counter, U);
+  } else {
+    double o[3] = {0, 0, 0};
+    if (origin)
+      for (int d = 0; d < 3; ++d) o[d] = origin[d];
+    hipLaunchKernelGGL(k_grav32, dim3(nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, o[0], o[1], o[2],
+                       F, ws.partials, ws.counter, U);
+  }
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,506 @@
+// Cell binning, stable counting sort by cell, and the Verlet (ELL) neighbour list.
+//
+// Follows accusim/forces.py:279-321 (cell_list_pairs) and 78-98 (VerletList):
+//   cell = clamp(trunc(((pos - lo) / span) * ncell), 0, ncell - 1), flat C-order index,
+//   atoms stably sorted by flat cell (argsort kind="stable"),
+//   27-cell stencil (wrapped if periodic, clipped if not, de-duplicated),
+//   pair kept when the minimum-image |d|^2 < rc^2, d = pos[j] - pos[i].
+// Every float64 expression that decides membership is evaluated in the reference's
+// operation order without contraction, so pair sets are bit-exact.
+//
+// Layout: the list is "ELL" by sorted slot t: nbr[k * stride + t] = atom id of the
+// k-th neighbour of atom sorted[t], k < nn[t]. Both directions are stored (full list),
+// so force kernels gather and never scatter. stride = n rounded up to 64.
+#include "common.h"
+#include "nlist.h"
+#include "../../include/accusim_hip.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace accu {
+
+// ------------------------------------------------------------------ kernels
+// Every rebuild kernel takes `flag`: when non-null and *flag == 0 the kernel exits,
+// so the whole chain can sit in a captured graph and run only when a rebuild is due.
+
+__global__ void k_zero_int(int* __restrict__ a, long long n, const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+__global__ void k_cell_assign(const double* __restrict__ pos, int stride_pos, int n, CellGeo g,
+                              int* __restrict__ cell, int* __restrict__ count,
+                              const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double t;
+    {
+#pragma clang fp contract(off)
+      t = ((pos[(size_t)stride_pos * i + d] - g.lo[d]) / g.span[d]) * (double)g.nc[d];
+    }
+    int m = g.nc[d] - 1;
+    int ci;
+    if (!(t >= 0.0)) ci = 0;  // negative (trunc then max 0) or NaN
+    else if (t >= (double)m) ci = m;
+    else ci = (int)t;
+    c[d] = ci;
+  }
+  int flat = (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2];
+  cell[i] = flat;
+  atomicAdd(&count[flat], 1);
+}
+
+// Exclusive scan of n ints with one 1024-thread block; out[n] = total.
+__global__ void __launch_bounds__(1024) k_scan_excl(const int* __restrict__ in, int* __restrict__ out,
+                                                    int* __restrict__ out2, long long n,
+                                                    const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  __shared__ long long s[1024];
+  const int t = threadIdx.x;
+  const long long per = (n + 1023) / 1024;
+  const long long a = t * per, b = (a + per < n) ? a + per : n;
+  long long sum = 0;
+  for (long long i = a; i < b; ++i) sum += in[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    long long v = (t >= o) ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  long long run = s[t] - sum;  // exclusive prefix of this chunk
+  for (long long i = a; i < b; ++i) {
+    int v = in[i];
+    out[i] = (int)run;
+    if (out2) out2[i] = (int)run;
+    run += v;
+  }
+  if (t == 1023) {
+    out[n] = (int)s[1023];
+    if (out2) out2[n] = (int)s[1023];
+  }
+}
+
+__global__ void k_scan_excl64(const int* __restrict__ in, long long* __restrict__ out, long long n) {
+  __shared__ long long s[1024];
+  const int t = threadIdx.x;
+  const long long per = (n + 1023) / 1024;
+  const long long a = t * per, b = (a + per < n) ? a + per : n;
+  long long sum = 0;
+  for (long long i = a; i < b; ++i) sum += in[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    long long v = (t >= o) ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  long long run = s[t] - sum;
+  for (long long i = a; i < b; ++i) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (t == 1023) out[n] = s[1023];
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ cell, int n, int* __restrict__ cursor,
+                               int* __restrict__ tmp, const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int slot = atomicAdd(&cursor[cell[i]], 1);
+  tmp[slot] = i;
+}
+
+// Within each cell, order atoms by id (= numpy's stable argsort of the flat cell ids).
+// One wave per cell; rank of each element = number of smaller ids in the segment.
+__global__ void k_cell_sort(const int* __restrict__ start, long long ncells, const int* __restrict__ tmp,
+                            int* __restrict__ sorted, const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  long long c = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= ncells) return;
+  const int lane = threadIdx.x & 63;
+  const int a = start[c], b = start[c + 1], len = b - a;
+  for (int e = lane; e < len; e += 64) {
+    int v = tmp[a + e];
+    int r = 0;
+    for (int x = 0; x < len; ++x) r += (tmp[a + x] < v);
+    sorted[a + r] = v;
+  }
+}
+
+__device__ __forceinline__ void unflat(int f, const int* nc, int* c) {
+  c[2] = f % nc[2];
+  f /= nc[2];
+  c[1] = f % nc[1];
+  c[0] = f / nc[1];
+}
+
+// Build the full ELL list. Thread = sorted slot t, atom i = sorted[t] (or t when
+// sorted == nullptr, i.e. the caller has already put atoms in cell order).
+__global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int n, CellGeo g,
+                            double rc2, const int* __restrict__ cell, const int* __restrict__ start,
+                            const int* __restrict__ sorted, int* __restrict__ nn, int* __restrict__ nbr,
+                            int stride, int max_nbrs, int* __restrict__ maxcount,
+                            const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sorted ? sorted[t] : t;
+  const double xi = pos[(size_t)stride_pos * i], yi = pos[(size_t)stride_pos * i + 1],
+               zi = pos[(size_t)stride_pos * i + 2];
+  int ci[3];
+  unflat(cell[i], g.nc, ci);
+  int seen[27];
+  int nseen = 0;
+  int cnt = 0;
+  for (int o = 0; o < 27; ++o) {
+    int c[3] = {ci[0] + o / 9 - 1, ci[1] + (o / 3) % 3 - 1, ci[2] + o % 3 - 1};
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (g.periodic) {
+        c[d] = ((c[d] % g.nc[d]) + g.nc[d]) % g.nc[d];
+      } else if (c[d] < 0 || c[d] >= g.nc[d]) {
+        ok = false;
+      }
+    }
+    if (!ok) continue;
+    int f = (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2];
+    if (g.dedup) {
+      bool dup = false;
+      for (int q = 0; q < nseen; ++q) dup |= (seen[q] == f);
+      if (dup) continue;
+      seen[nseen++] = f;
+    }
+    const int a = start[f], b = start[f + 1];
+    for (int s = a; s < b; ++s) {
+      const int j = sorted ? sorted[s] : s;
+      if (j == i) continue;
+      double dx, dy, dz;
+      {
+#pragma clang fp contract(off)
+        dx = pos[(size_t)stride_pos * j] - xi;
+        dy = pos[(size_t)stride_pos * j + 1] - yi;
+        dz = pos[(size_t)stride_pos * j + 2] - zi;
+      }
+      if (g.periodic) {
+        dx = minimg_exact(dx, g.box[0]);
+        dy = minimg_exact(dy, g.box[1]);
+        dz = minimg_exact(dz, g.box[2]);
+      }
+      const double r2 = r2_np(dx, dy, dz);
+      if (r2 < rc2) {
+        if (cnt < max_nbrs) nbr[(size_t)cnt * stride + t] = j;
+        ++cnt;
+      }
+    }
+  }
+  // the stored count never exceeds the capacity (readers index nbr with it); the true
+  // maximum goes to maxcount so the host can detect an overflow and regrow
+  nn[t] = cnt < max_nbrs ? cnt : max_nbrs;
+  atomicMax(maxcount, cnt);
+}
+
+__global__ void k_count_half(int n, const int* __restrict__ sorted, const int* __restrict__ nn,
+                             const int* __restrict__ nbr, int stride, int* __restrict__ cnt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sorted ? sorted[t] : t;
+  int c = 0;
+  for (int k = 0; k < nn[t]; ++k) c += (nbr[(size_t)k * stride + t] > i);
+  cnt[t] = c;
+}
+
+__global__ void k_emit_half(int n, const int* __restrict__ sorted, const int* __restrict__ nn,
+                            const int* __restrict__ nbr, int stride, const long long* __restrict__ off,
+                            long long* __restrict__ iout, long long* __restrict__ jout) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sorted ? sorted[t] : t;
+  long long o = off[t];
+  for (int k = 0; k < nn[t]; ++k) {
+    int j = nbr[(size_t)k * stride + t];
+    if (j > i) {
+      iout[o] = i;
+      jout[o] = j;
+      ++o;
+    }
+  }
+}
+
+// max_i |minimum-image(pos_i - ref_i)|^2 (forces.py:90-93), as the bit pattern of a
+// non-negative double so atomicMax orders it.
+__global__ void k_maxdisp2(const double* __restrict__ pos, int sp, const double* __restrict__ ref, int sr,
+                           int n, Box box, unsigned long long* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      {
+#pragma clang fp contract(off)
+        d[k] = pos[(size_t)sp * i + k] - ref[(size_t)sr * i + k];
+      }
+      if (box.periodic) d[k] = minimg_exact(d[k], box.L[k]);
+    }
+    v = r2_np(d[0], d[1], d[2]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+// ------------------------------------------------------------------ host side
+
+static int realloc_ints(int** p, long long count) {
+  if (*p) HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  HIP_TRY(hipMalloc(p, count * sizeof(int)));
+  return ACCU_OK;
+}
+
+int NList::ensure(int natoms, long long cells) {
+  int rc;
+  if (natoms + 1 > cap_n) {
+    long long c = natoms + 1 + natoms / 4 + 64;
+    int** arrs[] = {&cell, &sorted, &tmp, &nn, &halfcnt};
+    for (int** a : arrs)
+      if ((rc = realloc_ints(a, c))) return rc;
+    cap_n = c;
+  }
+  if (cells + 1 > cap_cells) {
+    long long c = cells + 1 + cells / 4 + 64;
+    int** arrs[] = {&count, &start, &cursor};
+    for (int** a : arrs)
+      if ((rc = realloc_ints(a, c))) return rc;
+    cap_cells = c;
+  }
+  if (!dev_scalars) {
+    HIP_TRY(hipMalloc(&dev_scalars, 64));
+    HIP_TRY(hipMemset(dev_scalars, 0, 64));
+  }
+  return ACCU_OK;
+}
+
+int NList::ensure_ell(int natoms, int nbrs) {
+  int st = (natoms + 63) / 64 * 64;
+  long long need = (long long)st * nbrs;
+  if (need > nbr_cap) {
+    if (nbr) HIP_TRY(hipFree(nbr));
+    HIP_TRY(hipMalloc(&nbr, need * sizeof(int)));
+    nbr_cap = need;
+  }
+  stride = st;
+  max_nbrs = nbrs;
+  return ACCU_OK;
+}
+
+void NList::release() {
+  int* ps[] = {cell, sorted, tmp, nn, halfcnt, count, start, cursor, nbr, dev_scalars};
+  for (int* p : ps)
+    if (p) (void)hipFree(p);
+  if (half_off) (void)hipFree(half_off);
+}
+
+// Run the binning + sort part of a rebuild (all kernels flag-gated).
+int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, const int* flag,
+                    hipStream_t st) {
+  const int T = 256;
+  const long long nc = h.ncells;
+  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nc + 1 + T - 1) / T)), dim3(T), 0, st, h.count, nc + 1, flag);
+  hipLaunchKernelGGL(k_cell_assign, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, h.cell, h.count,
+                     flag);
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, h.count, h.start, h.cursor, nc, flag);
+  hipLaunchKernelGGL(k_cell_scatter, dim3((n + T - 1) / T), dim3(T), 0, st, h.cell, n, h.cursor, h.tmp, flag);
+  hipLaunchKernelGGL(k_cell_sort, dim3((unsigned)((nc + 3) / 4)), dim3(256), 0, st, h.start, nc, h.tmp, h.sorted,
+                     flag);
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+int launch_build_ell(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, double rc2,
+                     const int* sorted, const int* flag, hipStream_t st) {
+  const int T = 256;
+  hipLaunchKernelGGL(k_build_ell, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, rc2, h.cell, h.start,
+                     sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, flag);
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+int make_geo(CellGeo& g, const double* lo, const double* span, const int* ncell, const double* box, int periodic) {
+  long long tot = 1;
+  g.dedup = 0;
+  for (int d = 0; d < 3; ++d) {
+    ARG_CHECK(ncell[d] >= 1, "ncell[%d] = %d must be >= 1", d, ncell[d]);
+    ARG_CHECK(span[d] > 0, "span[%d] must be positive", d);
+    g.lo[d] = lo[d];
+    g.span[d] = span[d];
+    g.nc[d] = ncell[d];
+    g.box[d] = box ? box[d] : 0.0;
+    tot *= ncell[d];
+    if (ncell[d] < 3) g.dedup = 1;
+  }
+  ARG_CHECK(tot < (1LL << 30), "too many cells (%lld)", tot);
+  ARG_CHECK(!periodic || box, "periodic list needs a box");
+  g.periodic = periodic;
+  return ACCU_OK;
+}
+
+}  // namespace accu
+
+using namespace accu;
+
+extern "C" {
+
+int accu_nlist_create(accu_nlist** out) {
+  ARG_CHECK(out, "null handle pointer");
+  *out = reinterpret_cast<accu_nlist*>(new NList());
+  return ACCU_OK;
+}
+
+int accu_nlist_destroy(accu_nlist* h) {
+  if (h) {
+    NList* p = reinterpret_cast<NList*>(h);
+    p->release();
+    delete p;
+  }
+  return ACCU_OK;
+}
+
+int accu_nlist_build(accu_nlist* hh, const double* pos, int n, const double* lo, const double* span,
+                     const int* ncell, const double* box, int periodic, double rc, void* stream) {
+  ARG_CHECK(hh && pos && lo && span && ncell, "null argument");
+  ARG_CHECK(n >= 0, "n must be >= 0");
+  NList& h = *reinterpret_cast<NList*>(hh);
+  hipStream_t st = (hipStream_t)stream;
+  CellGeo g;
+  int rc_ = make_geo(g, lo, span, ncell, box, periodic);
+  if (rc_) return rc_;
+  h.geo = g;
+  h.n = n;
+  h.ncells = (long long)ncell[0] * ncell[1] * ncell[2];
+  {
+#pragma clang fp contract(off)
+    h.rc2 = rc * rc;  // forces.py:320 compares with rc ** 2
+  }
+  if ((rc_ = h.ensure(n, h.ncells))) return rc_;
+  if (n == 0) {
+    h.last_max = 0;
+    return ACCU_OK;
+  }
+  if ((rc_ = launch_bin_sort(h, pos, 3, n, g, nullptr, st))) return rc_;
+  int guess = h.max_nbrs > 0 ? h.max_nbrs : 64;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if ((rc_ = h.ensure_ell(n, guess))) return rc_;
+    HIP_TRY(hipMemsetAsync(h.dev_scalars, 0, sizeof(int), st));
+    if ((rc_ = launch_build_ell(h, pos, 3, n, g, h.rc2, h.sorted, nullptr, st))) return rc_;
+    int mx = 0;
+    HIP_TRY(hipMemcpyAsync(&mx, h.dev_scalars, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    h.last_max = mx;
+    if (mx <= guess) return ACCU_OK;
+    guess = mx + mx / 8 + 8;
+  }
+  set_error("neighbour list did not fit after regrowing");
+  return ACCU_EOVERFLOW;
+}
+
+int accu_nlist_info(accu_nlist* hh, long long* out) {
+  ARG_CHECK(hh && out, "null argument");
+  NList& h = *reinterpret_cast<NList*>(hh);
+  out[0] = h.n;
+  out[1] = h.ncells;
+  out[2] = h.stride;
+  out[3] = h.max_nbrs;
+  out[4] = h.last_max;
+  out[5] = h.geo.nc[0];
+  out[6] = h.geo.nc[1];
+  out[7] = h.geo.nc[2];
+  return ACCU_OK;
+}
+
+int accu_nlist_cells(accu_nlist* hh, int* cell_out, int* sorted_out, int* nn_out, void* stream) {
+  ARG_CHECK(hh, "null handle");
+  NList& h = *reinterpret_cast<NList*>(hh);
+  hipStream_t st = (hipStream_t)stream;
+  if (h.n == 0) return ACCU_OK;
+  if (cell_out) HIP_TRY(hipMemcpyAsync(cell_out, h.cell, h.n * sizeof(int), hipMemcpyDeviceToDevice, st));
+  if (sorted_out) HIP_TRY(hipMemcpyAsync(sorted_out, h.sorted, h.n * sizeof(int), hipMemcpyDeviceToDevice, st));
+  if (nn_out) HIP_TRY(hipMemcpyAsync(nn_out, h.nn, h.n * sizeof(int), hipMemcpyDeviceToDevice, st));
+  return ACCU_OK;
+}
+
+int accu_nlist_count_half(accu_nlist* hh, long long* npairs, void* stream) {
+  ARG_CHECK(hh && npairs, "null argument");
+  NList& h = *reinterpret_cast<NList*>(hh);
+  hipStream_t st = (hipStream_t)stream;
+  *npairs = 0;
+  if (h.n == 0) return ACCU_OK;
+  if (!h.half_off || h.half_cap < h.n + 1) {
+    if (h.half_off) HIP_TRY(hipFree(h.half_off));
+    HIP_TRY(hipMalloc(&h.half_off, (h.n + 1) * sizeof(long long)));
+    h.half_cap = h.n + 1;
+  }
+  const int T = 256;
+  hipLaunchKernelGGL(k_count_half, dim3((h.n + T - 1) / T), dim3(T), 0, st, h.n, h.sorted, h.nn, h.nbr, h.stride,
+                     h.halfcnt);
+  hipLaunchKernelGGL(k_scan_excl64, dim3(1), dim3(1024), 0, st, h.halfcnt, h.half_off, (long long)h.n);
+  LAUNCH_CHECK();
+  HIP_TRY(hipMemcpyAsync(npairs, h.half_off + h.n, sizeof(long long), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return ACCU_OK;
+}
+
+int accu_nlist_export_half(accu_nlist* hh, long long* i_out, long long* j_out, void* stream) {
+  ARG_CHECK(hh && i_out && j_out, "null argument");
+  NList& h = *reinterpret_cast<NList*>(hh);
+  ARG_CHECK(h.half_off, "call accu_nlist_count_half first");
+  if (h.n == 0) return ACCU_OK;
+  const int T = 256;
+  hipLaunchKernelGGL(k_emit_half, dim3((h.n + T - 1) / T), dim3(T), 0, (hipStream_t)stream, h.n, h.sorted, h.nn,
+                     h.nbr, h.stride, h.half_off, i_out, j_out);
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+int accu_max_disp2(const double* pos, const double* ref, int n, const double* box, int periodic, double* out_host,
+                   void* stream) {
+  ARG_CHECK(pos && ref && out_host, "null argument");
+  ARG_CHECK(!periodic || box, "periodic needs a box");
+  hipStream_t st = (hipStream_t)stream;
+  *out_host = 0.0;
+  if (n <= 0) return ACCU_OK;
+  Workspace& ws = default_workspace();
+  int rc = ws.ensure(16);
+  if (rc) return rc;
+  Box b{};
+  b.periodic = periodic;
+  for (int d = 0; d < 3; ++d) {
+    b.L[d] = box ? box[d] : 1.0;
+    b.inv[d] = 1.0 / b.L[d];
+  }
+  unsigned long long* o = reinterpret_cast<unsigned long long*>(ws.out);
+  HIP_TRY(hipMemsetAsync(o, 0, sizeof(unsigned long long), st));
+  const int T = 256;
+  hipLaunchKernelGGL(k_maxdisp2, dim3((n + T - 1) / T), dim3(T), 0, st, pos, 3, ref, 3, n, b, o);
+  LAUNCH_CHECK();
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, o, sizeof bits, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  double v;
+  memcpy(&v, &bits, sizeof v);
+  *out_host = v;
+  return ACCU_OK;
+}
+
+}  // extern "C"

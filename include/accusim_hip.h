@@ -1,0 +1,123 @@
+/* C ABI of the MI355X (gfx950) force-path library for accusim.
+ *
+ * The reference (erezbz/DevinProj, package accusim) is pure Python/NumPy; its
+ * interface for this path is the duck-typed Force.compute(system) -> (F, U)
+ * (accusim/forces.py:1-21). The Python classes in devinproj_amd call the entry
+ * points below through ctypes; each comment names the reference function whose
+ * arithmetic the entry point replaces.
+ *
+ * Conventions
+ *   - Pointers named pos/F/q/mass/... are DEVICE pointers (float64 unless noted),
+ *     pos and F are [n][3] row-major like numpy arrays. Box/geometry arrays and
+ *     anything named *_host are host pointers.
+ *   - Return value 0 = ok; ACCU_EINVAL (-1) maps to ValueError, ACCU_ERUNTIME (-2)
+ *     and ACCU_EOVERFLOW (-3) to RuntimeError; accu_last_error() has the message.
+ *   - Work is enqueued on `stream` (a hipStream_t); outputs are caller-owned.
+ *   - U arguments are device float64 scalars and may be NULL (no energy).
+ *   - Handles are not thread-safe; the library keeps per-process scratch memory.
+ */
+#ifndef ACCUSIM_HIP_H
+#define ACCUSIM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACCU_ABI_VERSION 1
+
+enum { ACCU_FP64 = 0, ACCU_FP32_PAIR = 1 };
+
+typedef struct accu_nlist accu_nlist;
+
+const char* accu_last_error(void);
+int accu_version(void);
+int accu_device_info(char* name, int len, int* cus, long long* hbm_bytes);
+int accu_sync(void* stream);
+
+/* ---- cell list / Verlet list: forces.py:279-321 (cell_list_pairs), 78-98 (VerletList) */
+int accu_nlist_create(accu_nlist** out);
+int accu_nlist_destroy(accu_nlist* h);
+/* Bin atoms into ncell[0]*ncell[1]*ncell[2] cells of the region lo + [0, span)
+ * (forces.py:282-292; the caller computes lo, span, ncell exactly as the reference
+ * does), stable-sort them by cell, and store every pair with minimum-image
+ * |d|^2 < rc^2 in both directions. box may be NULL when periodic == 0. */
+int accu_nlist_build(accu_nlist* h, const double* pos, int n, const double* lo_host, const double* span_host,
+                     const int* ncell_host, const double* box_host, int periodic, double rc, void* stream);
+/* out[8] = n, ncells, stride, capacity per atom, max neighbours, ncell x, y, z */
+int accu_nlist_info(accu_nlist* h, long long* out_host);
+/* device int32 [n] copies of the cell id of each atom, the cell-sorted atom order and
+ * the per-slot neighbour counts; any pointer may be NULL */
+int accu_nlist_cells(accu_nlist* h, int* cell, int* sorted, int* nn, void* stream);
+/* number of i < j pairs (synchronises), then the pairs themselves as device int64 */
+int accu_nlist_count_half(accu_nlist* h, long long* npairs_host, void* stream);
+int accu_nlist_export_half(accu_nlist* h, long long* i_out, long long* j_out, void* stream);
+/* max_i |minimum image(pos_i - ref_i)|^2 (forces.py:90-93); synchronises */
+int accu_max_disp2(const double* pos, const double* ref, int n, const double* box_host, int periodic,
+                   double* out_host, void* stream);
+
+/* ---- Lennard-Jones: forces.py:119-142 (list, h != NULL) and 157-175 (all pairs, h == NULL).
+ * Species tables are host [ns][ns]: eps, sig^2, (cutoff*sig)^2; shift = src6^2 - src6.
+ * sp: device int32 [n] species index. precision: ACCU_FP64 or ACCU_FP32_PAIR. */
+int accu_lj_forces(accu_nlist* h, const double* pos, const int* sp, int n, int ns, const double* eps_host,
+                   const double* sig2_host, const double* rc2_host, double shift, const double* box_host,
+                   int periodic, int precision, double* F, double* U, void* stream);
+
+/* ---- DampedShiftedCoulomb (shifted = 1): forces.py:408-452; Ewald real space
+ * (shifted = 0): forces.py:371-379 plus the self term of forces.py:387.
+ * erc = erfc(alpha * rcut) from the caller; ex_keys: device sorted int64 lo*n+hi of
+ * excluded index pairs (nex may be 0). F and U are multiplied by k. */
+int accu_coulomb_forces(accu_nlist* h, const double* pos, const double* q, int n, int shifted, double alpha,
+                        double rcut, double erc, double k, const long long* ex_keys, int nex,
+                        const double* box_host, int periodic, int precision, double* F, double* U, void* stream);
+
+/* ---- Ewald reciprocal space: forces.py:380-386. cols_host: int32 [ncol][4] =
+ * (nx, ny, nz_lo, nz_hi), the k-vectors of one half space; accumulate != 0 adds to F.
+ * F += k * recip force, U = k * U_rec. */
+int accu_ewald_recip(const double* pos, const double* q, int n, const double* box_host, double alpha,
+                     const int* cols_host, int ncol, double k, int accumulate, double* F, double* U, void* stream);
+
+/* ---- Gravity: forces.py:38-48; with targets (device int64 [ntgt]) the rows of
+ * tree.py:81-87 direct_subset (F is then [ntgt][3]). origin (host [3], may be NULL)
+ * is subtracted before float32 rounding in ACCU_FP32_PAIR mode. */
+int accu_gravity_direct(const double* pos, const double* mass, int n, const long long* targets, int ntgt, double G,
+                        double eps2, const double* box_host, int periodic, int precision, const double* origin_host,
+                        double* F, double* U, void* stream);
+
+/* ---- Barnes-Hut: tree.py:45-78 (tree built on the host from pos_host/m_host,
+ * walked on the device). quadrupole != 0 adds quadrupole terms (extension). */
+int accu_bh_forces(const double* pos_host, const double* m_host, int n, double G, double eps2, double theta,
+                   int leaf_size, int quadrupole, double* F, double* U, long long* nnodes_out, void* stream);
+
+/* ---- device-resident MD: Simulation.run (simulation.py:95-112) driving
+ * VelocityVerlet / Yoshida4 (integrators.py:19-47) over LJ (forces.py:119-142) and/or
+ * DSF (forces.py:408-438), with the Verlet rebuild rule of forces.py:86-98 applied
+ * on the device each step and the record() sums of simulation.py:76-93 (PE, KE,
+ * momentum) reduced in float64 on the device.
+ * create: host arrays in user order, species tables (mass, charge) host [ns]. */
+typedef struct accu_md accu_md;
+int accu_md_create(accu_md** out, int n, const double* pos_host, const double* vel_host, const int* sp_host, int ns,
+                   const double* mass_host, const double* charge_host, const double* box_host, int periodic);
+int accu_md_destroy(accu_md* h);
+int accu_md_set_lj(accu_md* h, int ns, const double* eps_host, const double* sig2_host, const double* rc2_host,
+                   double shift);
+int accu_md_set_dsf(accu_md* h, double alpha, double rcut, double erc, double k);
+/* rc_list includes the skin; capacity = max neighbours at the first build * (1 + slack) + 16 */
+int accu_md_init(accu_md* h, double rc_list, double skin, int precision, double slack, int log_cap);
+/* steps steps of dt split into substeps w_host[0..nsub) * dt; a record every
+ * record_every steps and at the last; use_graph replays one captured hipGraph */
+int accu_md_run(accu_md* h, int steps, double dt, const double* w_host, int nsub, int record_every, int use_graph);
+int accu_md_sync(accu_md* h);
+/* state in user order (host [n][3], any may be NULL); log_host: [records][8] =
+ * PE, KE, Px, Py, Pz, rebuilds so far, 0, 0; stats: rebuilds, records, max
+ * neighbours seen, capacity, list entries. Returns ACCU_EOVERFLOW if the list capacity was exceeded. */
+int accu_md_get(accu_md* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
+                long long* stats);
+int accu_md_stream(accu_md* h, void** stream_out);
+/* HIP-event timing of every fused launch of host-driven runs: total ms and launches */
+int accu_md_set_timing(accu_md* h, int enable);
+int accu_md_timing(accu_md* h, double* total_ms_host, long long* launches_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
