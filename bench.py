@@ -37,13 +37,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_lj(s, precision, steps, warmup, record_every=1):
+def gpu_lj(s, precision, steps, warmup, record_every=1, graph_steps=0):
     import torch
 
     import devinproj_amd as dp
     from devinproj_amd import workloads as W
     eng = dp.MDEngine(s, [dp.LennardJones(W.argon_params(), cutoff=2.5, skin=0.3, precision=precision)],
-                      dp.VelocityVerlet(), dt=DT, precision=precision, log_cap=steps + warmup + 16)
+                      dp.VelocityVerlet(), dt=DT, precision=precision, log_cap=steps + warmup + graph_steps + 64)
     eng.run(warmup, record_every=record_every)
     eng.set_timing(True)
     torch.cuda.synchronize()
@@ -54,6 +54,7 @@ def gpu_lj(s, precision, steps, warmup, record_every=1):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms, launches = eng.timing()
+    eng.set_timing(False)
     eng.fetch(state=False)
     out = {
         "steps_per_s": steps / (t1 - t0),
@@ -64,6 +65,17 @@ def gpu_lj(s, precision, steps, warmup, record_every=1):
         "list_entries": eng.stats["list_entries"],
         "energy_drift": eng.energy_drift(),
     }
+    # the same run continued as replays of one captured hipGraph (blocks of 10 steps)
+    if graph_steps:
+        eng.use_graph = True
+        blk = 10
+        eng.run(blk, record_every=record_every)     # capture + first replay
+        t0 = time.perf_counter()
+        for _ in range(graph_steps // blk):
+            eng.run(blk, record_every=record_every, sync=False)
+        eng.sync()
+        t1 = time.perf_counter()
+        out["graph_steps_per_s"] = (graph_steps // blk) * blk / (t1 - t0)
     eng.close()
     return out
 
@@ -139,12 +151,12 @@ def main():
     npairs = pairs_within(s, rc)
     log(f"pairs within rc: {npairs}")
     log("mixed precision run")
-    mixed = gpu_lj(s.copy(), "mixed", args.steps, args.warmup)
+    mixed = gpu_lj(s.copy(), "mixed", args.steps, args.warmup, graph_steps=args.steps)
     log(f"mixed: {mixed}")
     fp64 = None
     if not args.no_fp64:
         log("fp64 run")
-        fp64 = gpu_lj(s.copy(), "fp64", args.steps, args.warmup)
+        fp64 = gpu_lj(s.copy(), "fp64", args.steps, args.warmup, graph_steps=args.steps)
         log(f"fp64: {fp64}")
     nacl = None
     if not args.no_dsf:
@@ -175,6 +187,8 @@ def main():
         "config": {"workload": "lj-argon-fcc-1048576-vv-dt10fs", "atoms": n, "cutoff_sigma": 2.5,
                    "skin_sigma": 0.3, "precision": "mixed", "record_every": 1, "device": name},
         "pair_interactions_per_s": npairs * mixed["steps_per_s"],
+        "graph_steps_per_s": {"mixed": mixed.get("graph_steps_per_s"),
+                              "fp64": fp64.get("graph_steps_per_s") if fp64 else None},
         "fp64_steps_per_s": fp64["steps_per_s"] if fp64 else None,
         "mixed_over_fp64": mixed["steps_per_s"] / fp64["steps_per_s"] if fp64 else None,
         "rebuilds": {"mixed": mixed["rebuilds"], "fp64": fp64["rebuilds"] if fp64 else None},

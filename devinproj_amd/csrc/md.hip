@@ -50,7 +50,55 @@ __device__ __forceinline__ double np_mod(double x, double L) {
   return r;
 }
 
-template <typename P, bool LJ_ON, bool COUL_ON>
+// Accumulators of one atom's pair sums: float64 totals plus float32 partials that the
+// mixed path flushes into the totals after every batch of neighbours.
+struct PairAcc {
+  double fx = 0, fy = 0, fz = 0, u = 0;   // LJ
+  double cx = 0, cy = 0, cz = 0, cu = 0;  // Coulomb (times k at the end)
+  float gx = 0.f, gy = 0.f, gz = 0.f, gu = 0.f, hx = 0.f, hy = 0.f, hz = 0.f, hu = 0.f;
+  __device__ __forceinline__ void flush() {
+    fx += gx; fy += gy; fz += gz; u += gu;
+    cx += hx; cy += hy; cz += hz; cu += hu;
+    gx = gy = gz = gu = hx = hy = hz = hu = 0.f;
+  }
+};
+
+// One neighbour j (position xj, species in xj.w) of atom i.
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
+__device__ __forceinline__ void md_pair(const MDParams& p, const LJ<P>& lj, const double4& xi, int row, double qi,
+                                        const double4& xj, PairAcc& a) {
+  double dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+  if (p.box.periodic) {
+    dx = minimg_fast(dx, p.box.L[0], p.box.inv[0]);
+    dy = minimg_fast(dy, p.box.L[1], p.box.inv[1]);
+    dz = minimg_fast(dz, p.box.L[2], p.box.inv[2]);
+  }
+  const int tj = ONE_SP ? 0 : (int)xj.w;
+  if constexpr (sizeof(P) == 8) {
+    const double r2 = r2_np(dx, dy, dz);
+    if constexpr (LJ_ON) {
+      if constexpr (ONE_SP) lj_pair64(dx, dy, dz, r2, p.lj.eps[0], p.lj.sig2[0], p.lj.rc2[0], p.lj.shift, a.fx, a.fy, a.fz, a.u);
+      else lj_pair64(dx, dy, dz, r2, lj.eps[row + tj], lj.sig2[row + tj], lj.rc2[row + tj], lj.shift, a.fx, a.fy, a.fz, a.u);
+    }
+    if constexpr (COUL_ON) coul_pair64(p.coul, dx, dy, dz, r2, qi * p.charge[tj], a.cx, a.cy, a.cz, a.cu);
+  } else {
+    const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
+    const float r2 = r2_npf(fdx, fdy, fdz);
+    if constexpr (LJ_ON) {
+      if constexpr (ONE_SP)
+        lj_pair32(fdx, fdy, fdz, r2, (float)p.lj.eps[0], (float)p.lj.sig2[0], (float)p.lj.rc2[0], (float)p.lj.shift,
+                  a.gx, a.gy, a.gz, a.gu);
+      else
+        lj_pair32(fdx, fdy, fdz, r2, lj.eps[row + tj], lj.sig2[row + tj], lj.rc2[row + tj], lj.shift, a.gx, a.gy, a.gz,
+                  a.gu);
+    }
+    if constexpr (COUL_ON)
+      coul_pair32((float)p.coul.alpha, (float)p.coul.rc, (float)p.coul.c, (float)p.coul.erc_rc, (float)p.coul.fshift,
+                  p.coul.shifted, fdx, fdy, fdz, r2, (float)(qi * p.charge[tj]), a.hx, a.hy, a.hz, a.hu);
+  }
+}
+
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 __global__ void __launch_bounds__(kMT) k_md_fused(MDParams p, const double4* __restrict__ pos,
                                                   double4* __restrict__ pos_next, double4* __restrict__ vel,
                                                   double4* __restrict__ Fst, const int* __restrict__ sp,
@@ -62,7 +110,7 @@ __global__ void __launch_bounds__(kMT) k_md_fused(MDParams p, const double4* __r
   __shared__ double lds[3 * kMaxSpecies * kMaxSpecies];
   __shared__ double red[kLogW * 4];
   LJ<P> lj;
-  if constexpr (LJ_ON) {
+  if constexpr (LJ_ON && !ONE_SP) {
     lj.sp = sp;
     lj.stage(p.lj, lds);
   }
@@ -72,51 +120,29 @@ __global__ void __launch_bounds__(kMT) k_md_fused(MDParams p, const double4* __r
   bool moved_far = false;
   if (i < p.n) {
     const double4 xi = pos[i];
-    const int ti = sp[i];
+    const int ti = ONE_SP ? 0 : (int)xi.w;
     const int row = ti * p.ns;
     const double qi = p.charge[ti];
     const int cnt = nn[i];
-    double fx = 0, fy = 0, fz = 0, u = 0;       // LJ
-    double cx = 0, cy = 0, cz = 0, cu = 0;      // Coulomb (scaled by k at the end)
-    float gx = 0.f, gy = 0.f, gz = 0.f, gu = 0.f, hx = 0.f, hy = 0.f, hz = 0.f, hu = 0.f;
-    for (int k = 0; k < cnt; ++k) {
-      const int j = nbr[(size_t)k * stride + i];
-      const double4 xj = pos[j];
-      double dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-      if (p.box.periodic) {
-        dx = minimg_fast(dx, p.box.L[0], p.box.inv[0]);
-        dy = minimg_fast(dy, p.box.L[1], p.box.inv[1]);
-        dz = minimg_fast(dz, p.box.L[2], p.box.inv[2]);
-      }
-      const int tj = sp[j];
-      if constexpr (sizeof(P) == 8) {
-        const double r2 = r2_np(dx, dy, dz);
-        if constexpr (LJ_ON) {
-          const int pj = row + tj;
-          lj_pair64(dx, dy, dz, r2, lj.eps[pj], lj.sig2[pj], lj.rc2[pj], lj.shift, fx, fy, fz, u);
-        }
-        if constexpr (COUL_ON) coul_pair64(p.coul, dx, dy, dz, r2, qi * p.charge[tj], cx, cy, cz, cu);
-      } else {
-        const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
-        const float r2 = r2_npf(fdx, fdy, fdz);
-        if constexpr (LJ_ON) {
-          const int pj = row + tj;
-          lj_pair32(fdx, fdy, fdz, r2, lj.eps[pj], lj.sig2[pj], lj.rc2[pj], lj.shift, gx, gy, gz, gu);
-        }
-        if constexpr (COUL_ON)
-          coul_pair32((float)p.coul.alpha, (float)p.coul.rc, (float)p.coul.c, (float)p.coul.erc_rc,
-                      (float)p.coul.fshift, p.coul.shifted, fdx, fdy, fdz, r2, (float)(qi * p.charge[tj]), hx, hy,
-                      hz, hu);
-        // flush the float32 partial sums into float64 every 8 neighbours
-        if ((k & 7) == 7) {
-          fx += gx; fy += gy; fz += gz; u += gu;
-          cx += hx; cy += hy; cz += hz; cu += hu;
-          gx = gy = gz = gu = hx = hy = hz = hu = 0.f;
-        }
-      }
+    const int* __restrict__ col = nbr + i;
+    PairAcc a;
+    // batches of 4 neighbours: the four index loads, then the four position gathers,
+    // are independent, so each lane keeps several memory requests in flight
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+      const int j0 = col[(size_t)k * stride], j1 = col[(size_t)(k + 1) * stride];
+      const int j2 = col[(size_t)(k + 2) * stride], j3 = col[(size_t)(k + 3) * stride];
+      const double4 x0 = pos[j0], x1 = pos[j1], x2 = pos[j2], x3 = pos[j3];
+      md_pair<P, LJ_ON, COUL_ON, ONE_SP>(p, lj, xi, row, qi, x0, a);
+      md_pair<P, LJ_ON, COUL_ON, ONE_SP>(p, lj, xi, row, qi, x1, a);
+      md_pair<P, LJ_ON, COUL_ON, ONE_SP>(p, lj, xi, row, qi, x2, a);
+      md_pair<P, LJ_ON, COUL_ON, ONE_SP>(p, lj, xi, row, qi, x3, a);
+      if constexpr (sizeof(P) == 4) a.flush();
     }
-    fx += gx; fy += gy; fz += gz; u += gu;
-    cx += hx; cy += hy; cz += hz; cu += hu;
+    for (; k < cnt; ++k) md_pair<P, LJ_ON, COUL_ON, ONE_SP>(p, lj, xi, row, qi, pos[col[(size_t)k * stride]], a);
+    a.flush();
+    double fx = a.fx, fy = a.fy, fz = a.fz, u = a.u;
+    double cx = a.cx, cy = a.cy, cz = a.cz, cu = a.cu;
     if constexpr (COUL_ON) {
       cu -= qi * qi * p.coul.self;
       fx += p.coul.k * cx;
@@ -374,23 +400,32 @@ struct MD {
     if (st) (void)hipStreamDestroy(st);
   }
 
-  template <typename P>
+  template <typename P, bool ONE>
   void launch_fused_t(const double4* x, double4* xn, double hc, double hn, double dtn, int drift, int rec) {
     const int nb = (p.n + kMT - 1) / kMT;
     if (p.has_lj && p.has_coul)
-      hipLaunchKernelGGL((k_md_fused<P, true, true>), dim3(nb), dim3(kMT), 0, st, p, x, xn, vel, F, sp, ref, nl.nn,
-                         nl.nbr, nl.stride, hc, hn, dtn, drift, rec, ints, partials, counter, log, ints + 2, ints + 1);
+      hipLaunchKernelGGL((k_md_fused<P, true, true, ONE>), dim3(nb), dim3(kMT), 0, st, p, x, xn, vel, F, sp, ref,
+                         nl.nn, nl.nbr, nl.stride, hc, hn, dtn, drift, rec, ints, partials, counter, log, ints + 2,
+                         ints + 1);
     else if (p.has_lj)
-      hipLaunchKernelGGL((k_md_fused<P, true, false>), dim3(nb), dim3(kMT), 0, st, p, x, xn, vel, F, sp, ref, nl.nn,
-                         nl.nbr, nl.stride, hc, hn, dtn, drift, rec, ints, partials, counter, log, ints + 2, ints + 1);
+      hipLaunchKernelGGL((k_md_fused<P, true, false, ONE>), dim3(nb), dim3(kMT), 0, st, p, x, xn, vel, F, sp, ref,
+                         nl.nn, nl.nbr, nl.stride, hc, hn, dtn, drift, rec, ints, partials, counter, log, ints + 2,
+                         ints + 1);
     else
-      hipLaunchKernelGGL((k_md_fused<P, false, true>), dim3(nb), dim3(kMT), 0, st, p, x, xn, vel, F, sp, ref, nl.nn,
-                         nl.nbr, nl.stride, hc, hn, dtn, drift, rec, ints, partials, counter, log, ints + 2, ints + 1);
+      hipLaunchKernelGGL((k_md_fused<P, false, true, ONE>), dim3(nb), dim3(kMT), 0, st, p, x, xn, vel, F, sp, ref,
+                         nl.nn, nl.nbr, nl.stride, hc, hn, dtn, drift, rec, ints, partials, counter, log, ints + 2,
+                         ints + 1);
   }
 
   void launch_fused(const double4* x, double4* xn, double hc, double hn, double dtn, int drift, int rec) {
-    if (precision == ACCU_FP64) launch_fused_t<double>(x, xn, hc, hn, dtn, drift, rec);
-    else launch_fused_t<float>(x, xn, hc, hn, dtn, drift, rec);
+    const bool one = p.ns == 1;
+    if (precision == ACCU_FP64) {
+      if (one) launch_fused_t<double, true>(x, xn, hc, hn, dtn, drift, rec);
+      else launch_fused_t<double, false>(x, xn, hc, hn, dtn, drift, rec);
+    } else {
+      if (one) launch_fused_t<float, true>(x, xn, hc, hn, dtn, drift, rec);
+      else launch_fused_t<float, false>(x, xn, hc, hn, dtn, drift, rec);
+    }
   }
 
   // Flag-gated rebuild chain: bin, sort, permute state into cell order, rebuild the list.
@@ -401,7 +436,7 @@ struct MD {
                        F2, sp2, perm2, cell2, ints);
     hipLaunchKernelGGL(k_md_copyback, dim3(nb), dim3(T), 0, st, n, x, ref, vel, F, sp, perm, nl.cell, pos2, vel2, F2,
                        sp2, perm2, cell2, ints);
-    launch_build_ell(nl, reinterpret_cast<const double*>(x), 4, n, geo, nl.rc2, nullptr, ints, st);
+    launch_build_ell(nl, reinterpret_cast<const double*>(x), 4, n, geo, nl.rc2, nullptr, ints, st, 0);
     hipLaunchKernelGGL(k_md_end_rebuild, dim3(1), dim3(64), 0, st, ints, ints + 1);
   }
 
@@ -461,6 +496,8 @@ int accu_md_create(accu_md** out, int n, const double* pos_host, const double* v
   ARG_CHECK(n >= 2, "need at least 2 atoms");
   ARG_CHECK(ns >= 1 && ns <= kMaxSpecies, "number of species %d outside [1, %d]", ns, kMaxSpecies);
   ARG_CHECK(periodic && box_host, "the device MD engine needs a periodic box");
+  for (int i = 0; i < n; ++i)
+    ARG_CHECK(sp_host[i] >= 0 && sp_host[i] < ns, "species index %d of atom %d outside [0, %d)", sp_host[i], i, ns);
   MD* m = new MD();
   m->p.n = n;
   m->p.ns = ns;
@@ -485,7 +522,9 @@ int accu_md_create(accu_md** out, int n, const double* pos_host, const double* v
   std::vector<double4> h4(n);
   std::vector<int> perm(n);
   for (int i = 0; i < n; ++i) {
-    h4[i] = make_double4(pos_host[3 * (size_t)i], pos_host[3 * (size_t)i + 1], pos_host[3 * (size_t)i + 2], 0.0);
+    // the species index rides in pos.w so one gather brings position and type
+    h4[i] = make_double4(pos_host[3 * (size_t)i], pos_host[3 * (size_t)i + 1], pos_host[3 * (size_t)i + 2],
+                         (double)sp_host[i]);
     perm[i] = i;
   }
   HIP_TRY(hipMemcpy(m->pos, h4.data(), n * sizeof(double4), hipMemcpyHostToDevice));
@@ -581,7 +620,7 @@ int accu_md_init(accu_md* h, double rc_list, double skin, int precision, double 
   if ((rc = m.nl.ensure_ell(n, 1))) return rc;
   HIP_TRY(hipMemsetAsync(m.nl.dev_scalars, 0, sizeof(int), m.st));
   if ((rc = launch_build_ell(m.nl, reinterpret_cast<const double*>(m.pos), 4, n, m.geo, m.nl.rc2, nullptr, m.ints,
-                             m.st)))
+                             m.st, 0)))
     return rc;
   int mx = 0;
   HIP_TRY(hipMemcpyAsync(&mx, m.nl.dev_scalars, sizeof(int), hipMemcpyDeviceToHost, m.st));
@@ -589,7 +628,7 @@ int accu_md_init(accu_md* h, double rc_list, double skin, int precision, double 
   const int cap = (int)(mx * (1.0 + slack)) + 16;
   if ((rc = m.nl.ensure_ell(n, cap))) return rc;
   if ((rc = launch_build_ell(m.nl, reinterpret_cast<const double*>(m.pos), 4, n, m.geo, m.nl.rc2, nullptr, m.ints,
-                             m.st)))
+                             m.st, 0)))
     return rc;
   hipLaunchKernelGGL(k_md_end_rebuild, dim3(1), dim3(64), 0, m.st, m.ints, m.ints + 1);
   // forces at t = 0 and the first record (no kick, no drift)

@@ -49,7 +49,9 @@ struct NList {
 int make_geo(CellGeo& g, const double* lo, const double* span, const int* ncell, const double* box, int periodic);
 int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, const int* flag,
                     hipStream_t st);
+// exact = 1: minimum image with the true division (bit-exact pair sets, the drop-in
+// path); exact = 0: reciprocal multiply, identical for every |d| < L/2 (engine rebuilds)
 int launch_build_ell(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, double rc2,
-                     const int* sorted, const int* flag, hipStream_t st);
+                     const int* sorted, const int* flag, hipStream_t st, int exact = 1);
 
 }  // namespace accu

@@ -150,9 +150,10 @@ __global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int 
                             double rc2, const int* __restrict__ cell, const int* __restrict__ start,
                             const int* __restrict__ sorted, int* __restrict__ nn, int* __restrict__ nbr,
                             int stride, int max_nbrs, int* __restrict__ maxcount,
-                            const int* __restrict__ flag) {
+                            const int* __restrict__ flag, int exact) {
   if (flag && *flag == 0) return;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double inv[3] = {1.0 / g.box[0], 1.0 / g.box[1], 1.0 / g.box[2]};
   if (t >= n) return;
   const int i = sorted ? sorted[t] : t;
   const double xi = pos[(size_t)stride_pos * i], yi = pos[(size_t)stride_pos * i + 1],
@@ -193,9 +194,15 @@ __global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int 
         dz = pos[(size_t)stride_pos * j + 2] - zi;
       }
       if (g.periodic) {
-        dx = minimg_exact(dx, g.box[0]);
-        dy = minimg_exact(dy, g.box[1]);
-        dz = minimg_exact(dz, g.box[2]);
+        if (exact) {
+          dx = minimg_exact(dx, g.box[0]);
+          dy = minimg_exact(dy, g.box[1]);
+          dz = minimg_exact(dz, g.box[2]);
+        } else {
+          dx = minimg_fast(dx, g.box[0], inv[0]);
+          dy = minimg_fast(dy, g.box[1], inv[1]);
+          dz = minimg_fast(dz, g.box[2], inv[2]);
+        }
       }
       const double r2 = r2_np(dx, dy, dz);
       if (r2 < rc2) {
@@ -329,10 +336,10 @@ int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const Ce
 }
 
 int launch_build_ell(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, double rc2,
-                     const int* sorted, const int* flag, hipStream_t st) {
+                     const int* sorted, const int* flag, hipStream_t st, int exact) {
   const int T = 256;
   hipLaunchKernelGGL(k_build_ell, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, rc2, h.cell, h.start,
-                     sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, flag);
+                     sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, flag, exact);
   LAUNCH_CHECK();
   return ACCU_OK;
 }

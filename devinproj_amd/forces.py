@@ -82,10 +82,18 @@ class NeighbourList:
             self._h = h
         return self._h
 
+    MAX_CELLS = 1 << 22
+
     def build(self, s, pos_d, rc):
         if s.dim != 3:
             raise ValueError("the GPU neighbour list supports 3D systems")
         lo, span, ncell = cell_grid(s, rc)
+        if int(np.prod(ncell.astype(np.int64))) > self.MAX_CELLS:
+            # a dilute system: coarser cells (still >= rc wide) give the same pair set
+            f = int(np.ceil((np.prod(ncell.astype(float)) / self.MAX_CELLS) ** (1 / 3)))
+            ncell = np.maximum(ncell // f, 1).astype(np.int32)
+            if s.periodic:
+                ncell = np.where(ncell < 3, 1, ncell).astype(np.int32)
         box, periodic = _box_args(s)
         L.call("accu_nlist_build", self.handle, L.ptr(pos_d), s.n, L.ptr(lo), L.ptr(span), L.ptr(ncell),
                L.ptr(box), periodic, float(rc), L.stream())

@@ -94,10 +94,15 @@ def test_lj_matches_oracle_two_species(periodic, threshold):
 
 
 def test_lj_argon_864_matches_reference_golden(golden):
+    # at the perfect lattice the net forces are cancellation noise (1e-13 of a pair
+    # force), so compare there only the energy; forces at the state after 20 steps
     g = golden("argon864_vv20.npz")
     s = B.argon(6, seed=64)
+    _, U0 = dp.LennardJones(B.argon_params(), cutoff=2.5, skin=0.3).compute(s)
+    assert U0 == pytest.approx(float(g["U0"]), rel=1e-13)
+    s.pos = g["pos"].copy()
     Fg, Ug = dp.LennardJones(B.argon_params(), cutoff=2.5, skin=0.3).compute(s)
-    _close(Fg, Ug, g["F0"], float(g["U0"]), rtol=1e-12)
+    _close(Fg, Ug, g["F"], float(g["pe"][-1]), rtol=1e-11)
 
 
 def test_lj_mixed_precision_error_is_small():
