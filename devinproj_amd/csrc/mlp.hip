@@ -261,4 +261,452 @@ __global__ void __launch_bounds__(256) k_mlp_force(const double* __restrict__ po
   }
 }
 
+// ---------------------------------------------------------------- training
+
+// Per-wave LDS staging for the layout changes between the MFMA products (bf16).
+struct GradScratch {
+  __bf16 h1T[kH * 32];   // [v][p]
+  __bf16 dz2[kH * 32];   // [u][p]
+  __bf16 dz2T[32 * kH];  // [p][u]
+  __bf16 dz1[kH * 32];   // [v][p]
+  __bf16 xT[16 * 32];    // [c][p], c = 4 is the bias column
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Gradient of L = (1 / 3N) sum_i |(F_corr_i - target_i) / F0|^2 for one snapshot.
+// Each wave accumulates its atoms' contributions in registers (MFMA accumulators for
+// dW2 and dW1|db1) and writes one partial gradient vector; a second kernel sums the
+// partials in wave order, so the result does not depend on scheduling.
+//   ds      = dL/ds_ij = (2/3N) e_i . d_ij / r_ij * env(r_ij)
+//   dW2    += dz2 (u x p) h1 (p x v)          MFMA 32x32x16, K = pairs
+//   dh1     = W2^T (v x u) dz2 (u x p)         MFMA 32x32x16, K = hidden units
+//   dW1|db1 += dz1 (v x p) [x | 1] (p x c)     MFMA 16x16x32, K = pairs
+__global__ void __launch_bounds__(256) k_mlp_grad(const double* __restrict__ pos, int ps, int n,
+                                                  const int* __restrict__ sorted, const int* __restrict__ nn,
+                                                  const int* __restrict__ nbr, int stride, Box box, MLPGeom g,
+                                                  const float* __restrict__ prm, const __bf16* __restrict__ W2b,
+                                                  const __bf16* __restrict__ W2Tb, const float* __restrict__ q,
+                                                  const float* __restrict__ coord, const double* __restrict__ Fcorr,
+                                                  const double* __restrict__ target, double inv_m,
+                                                  float* __restrict__ gpart, double* __restrict__ lpart) {
+  __shared__ MLPShared sh;
+  __shared__ QEntry qall[4][kQ];
+  __shared__ GradScratch gs[4];
+  stage_params(prm, sh);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, p = lane & 31, hk = lane >> 5;
+  QEntry* qb = qall[w];
+  GradScratch& S = gs[w];
+  bf16x8 aW2[2][4], aW2T[2][4];
+  load_w2_operands(W2b, p, hk, aW2);
+  load_w2_operands(W2Tb, p, hk, aW2T);
+  f32x16 gW2[2][2];
+  f32x4 gW1[4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) gW2[a][b] = f32x16{};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) gW1[a] = f32x4{};
+  float gw3[32], gb2[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) gw3[k] = gb2[k] = 0.f;
+  float gb3 = 0.f;
+  double loss = 0.0;
+  const int wave_id = blockIdx.x * (blockDim.x >> 6) + w;
+  const int nwaves_total = gridDim.x * (blockDim.x >> 6);
+  const double invF0 = 1.0 / (double)g.F0;
+  for (int t = wave_id; t < n; t += nwaves_total) {
+    const int i = sorted ? sorted[t] : t;
+    const double xi = pos[(size_t)ps * i], yi = pos[(size_t)ps * i + 1], zi = pos[(size_t)ps * i + 2];
+    const float qi = q[i], ni = coord[i];
+    const double ex = (Fcorr[3 * (size_t)i] - target[3 * (size_t)i]) * invF0;
+    const double ey = (Fcorr[3 * (size_t)i + 1] - target[3 * (size_t)i + 1]) * invF0;
+    const double ez = (Fcorr[3 * (size_t)i + 2] - target[3 * (size_t)i + 2]) * invF0;
+    if (lane == 0) loss += (ex * ex + ey * ey + ez * ez) * inv_m;
+    const float gx = (float)(2.0 * ex * inv_m), gy = (float)(2.0 * ey * inv_m), gz = (float)(2.0 * ez * inv_m);
+    const int cnt = nn[t];
+    int qc = 0;
+    for (int base = 0; base < cnt || qc > 0; base += 64) {
+      if (base < cnt) qc = enqueue_chunk(base, cnt, i, t, pos, ps, nbr, stride, box, g, xi, yi, zi, qb, qc);
+      while (qc >= 32 || (base + 64 >= cnt && qc > 0)) {
+        const int m = qc < 32 ? qc : 32;
+        const bool valid = p < m;
+        const QEntry e = qb[valid ? p : 0];
+        TileState st;
+        tile_features(e, valid, qi, ni, q, coord, g, st);
+        tile_forward(sh, aW2, hk, st);
+        const float ds = valid ? (gx * e.dx + gy * e.dy + gz * e.dz) * __builtin_amdgcn_rcpf(st.r) * st.env : 0.f;
+        gb3 += (hk == 0) ? ds : 0.f;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int u = mb * 32 + 8 * (r >> 2) + 4 * hk + (r & 3);
+            const float z = st.z2[mb][r];
+            gw3[mb * 16 + r] = fmaf(ds, silu(z), gw3[mb * 16 + r]);
+            const float dz = ds * sh.w3[u] * dsilu(z);
+            gb2[mb * 16 + r] += dz;
+            const __bf16 bz = (__bf16)dz;
+            S.dz2[u * 32 + p] = bz;
+            S.dz2T[p * kH + u] = bz;
+          }
+        }
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+          for (int tt = 0; tt < 8; ++tt) S.h1T[(kb * 16 + hk * 8 + tt) * 32 + p] = st.bh1[kb][tt];
+        if (hk == 0) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) {
+            const float xv = !valid ? 0.f : (c < 4 ? st.x[c] : (c == 4 ? 1.f : 0.f));
+            S.xT[c * 32 + p] = (__bf16)xv;
+          }
+        }
+        wave_sync();
+        // dW2 += dz2 h1
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+          for (int mb = 0; mb < 2; ++mb) {
+            const bf16x8 A = *reinterpret_cast<const bf16x8*>(&S.dz2[(mb * 32 + p) * 32 + kb * 16 + hk * 8]);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+              const bf16x8 Bm = *reinterpret_cast<const bf16x8*>(&S.h1T[(nb * 32 + p) * 32 + kb * 16 + hk * 8]);
+              gW2[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bm, gW2[mb][nb], 0, 0, 0);
+            }
+          }
+        }
+        // dh1 = W2^T dz2, then dz1 = dh1 * silu'(z1) (z1 recomputed for the D-layout units)
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+          f32x16 dh = {};
+#pragma unroll
+          for (int kb = 0; kb < 4; ++kb) {
+            const bf16x8 Bm = *reinterpret_cast<const bf16x8*>(&S.dz2T[p * kH + kb * 16 + hk * 8]);
+            dh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aW2T[mb][kb], Bm, dh, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int v = mb * 32 + 8 * (r >> 2) + 4 * hk + (r & 3);
+            float z1 = sh.b1[v];
+            z1 = fmaf(sh.W1[4 * v], st.x[0], z1);
+            z1 = fmaf(sh.W1[4 * v + 1], st.x[1], z1);
+            z1 = fmaf(sh.W1[4 * v + 2], st.x[2], z1);
+            z1 = fmaf(sh.W1[4 * v + 3], st.x[3], z1);
+            S.dz1[v * 32 + p] = (__bf16)(dh[r] * dsilu(z1));
+          }
+        }
+        wave_sync();
+        // dW1 | db1 += dz1 [x | 1]
+#pragma unroll
+        for (int vb = 0; vb < 4; ++vb) {
+          const bf16x8 A = *reinterpret_cast<const bf16x8*>(&S.dz1[(vb * 16 + (lane & 15)) * 32 + (lane >> 4) * 8]);
+          const bf16x8 Bm = *reinterpret_cast<const bf16x8*>(&S.xT[(lane & 15) * 32 + (lane >> 4) * 8]);
+          gW1[vb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, gW1[vb], 0, 0, 0);
+        }
+        wave_sync();
+        const int rest = qc - m;
+        QEntry mv{};
+        if (lane < rest) mv = qb[m + lane];
+        wave_sync();
+        if (lane < rest) qb[lane] = mv;
+        wave_sync();
+        qc = rest;
+      }
+    }
+  }
+  // one partial gradient per wave
+  float* gp = gpart + (size_t)wave_id * kNParam;
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int u = mb * 32 + 8 * (r >> 2) + 4 * hk + (r & 3), v = nb * 32 + p;
+        gp[kOffW2 + u * kH + v] = gW2[mb][nb][r];
+      }
+#pragma unroll
+  for (int vb = 0; vb < 4; ++vb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int v = vb * 16 + (lane >> 4) * 4 + r, c = lane & 15;
+      if (c < 4) gp[kOffW1 + v * 4 + c] = gW1[vb][r];
+      else if (c == 4) gp[kOffB1 + v] = gW1[vb][r];
+    }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    float a = gw3[k], b = gb2[k];
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      a += __shfl_xor(a, o, 64);
+      b += __shfl_xor(b, o, 64);
+    }
+    if (p == 0) {
+      const int mb = k >> 4, r = k & 15;
+      const int u = mb * 32 + 8 * (r >> 2) + 4 * hk + (r & 3);
+      gp[kOffW3 + u] = a;
+      gp[kOffB2 + u] = b;
+    }
+  }
+  const double b3 = wave_sum((double)gb3);
+  if (lane == 0) {
+    gp[kOffB3] = (float)b3;
+    lpart[wave_id] = loss;
+  }
+}
+
+// grad[k] = sum over waves (fixed order); loss likewise.
+__global__ void k_mlp_reduce(const float* __restrict__ gpart, const double* __restrict__ lpart, int nw,
+                             float* __restrict__ grad, double* __restrict__ loss) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < kNParam) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += gpart[(size_t)w * kNParam + k];
+    grad[k] = (float)s;
+  }
+  if (k == kNParam) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += lpart[w];
+    *loss = s;
+  }
+}
+
+// Adam on the float32 master parameters; refreshes the bf16 copies of W2 and W2^T.
+__global__ void k_mlp_adam(float* __restrict__ prm, float* __restrict__ m1, float* __restrict__ m2,
+                           const float* __restrict__ grad, float lr, float b1, float b2, float eps, float c1, float c2,
+                           __bf16* __restrict__ W2b, __bf16* __restrict__ W2Tb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kNParam) return;
+  const float gk = grad[k];
+  const float a = b1 * m1[k] + (1.f - b1) * gk;
+  const float b = b2 * m2[k] + (1.f - b2) * gk * gk;
+  m1[k] = a;
+  m2[k] = b;
+  const float v = prm[k] - lr * (a * c1) / (sqrtf(b * c2) + eps);
+  prm[k] = v;
+  if (k >= kOffW2 && k < kOffW2 + kH * kH) {
+    const int u = (k - kOffW2) / kH, c = (k - kOffW2) % kH;
+    W2b[u * kH + c] = (__bf16)v;
+    W2Tb[c * kH + u] = (__bf16)v;
+  }
+}
+
+__global__ void k_mlp_sync_w2(const float* __restrict__ prm, __bf16* __restrict__ W2b, __bf16* __restrict__ W2Tb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kH * kH) return;
+  const int u = k / kH, c = k % kH;
+  const __bf16 v = (__bf16)prm[kOffW2 + k];
+  W2b[u * kH + c] = v;
+  W2Tb[c * kH + u] = v;
+}
+
+__global__ void k_to_float(const double* __restrict__ a, double scale, int n, float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (float)(a[k] * scale);
+}
+
+struct MLP {
+  float* prm = nullptr;  // kNParam
+  float* m1 = nullptr;
+  float* m2 = nullptr;
+  float* grad = nullptr;
+  __bf16* W2b = nullptr;
+  __bf16* W2Tb = nullptr;
+  float* q = nullptr;
+  float* coord = nullptr;
+  double* Fcorr = nullptr;
+  int cap_n = 0;
+  float* gpart = nullptr;
+  double* lpart = nullptr;
+  double* loss = nullptr;
+  int nwaves = 0;
+  int step = 0;
+  MLPGeom g{};
+
+  int ensure(int n) {
+    if (n > cap_n) {
+      if (q) HIP_TRY(hipFree(q));
+      if (coord) HIP_TRY(hipFree(coord));
+      if (Fcorr) HIP_TRY(hipFree(Fcorr));
+      HIP_TRY(hipMalloc(&q, (size_t)n * sizeof(float)));
+      HIP_TRY(hipMalloc(&coord, (size_t)n * sizeof(float)));
+      HIP_TRY(hipMalloc(&Fcorr, (size_t)n * 3 * sizeof(double)));
+      cap_n = n;
+    }
+    return ACCU_OK;
+  }
+  void release() {
+    void* ps[] = {prm, m1, m2, grad, W2b, W2Tb, q, coord, Fcorr, gpart, lpart, loss};
+    for (void* x : ps)
+      if (x) (void)hipFree(x);
+  }
+};
+
 }  // namespace accu
+
+using namespace accu;
+
+static Box box_of(const double* box, int periodic) {
+  Box b{};
+  b.periodic = periodic && box;
+  for (int d = 0; d < 3; ++d) {
+    b.L[d] = box ? box[d] : 1.0;
+    b.inv[d] = 1.0 / b.L[d];
+  }
+  return b;
+}
+
+extern "C" {
+
+int accu_mlp_create(accu_mlp** out, const float* params_host) {
+  ARG_CHECK(out && params_host, "null argument");
+  MLP* m = new MLP();
+  auto fail = [&](hipError_t e) {
+    set_error("hipMalloc: %s", hipGetErrorString(e));
+    m->release();
+    delete m;
+    return ACCU_ERUNTIME;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&m->prm, kNParam * sizeof(float))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&m->m1, kNParam * sizeof(float))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&m->m2, kNParam * sizeof(float))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&m->grad, kNParam * sizeof(float))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&m->W2b, kH * kH * sizeof(__bf16))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&m->W2Tb, kH * kH * sizeof(__bf16))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&m->loss, 64)) != hipSuccess) return fail(e);
+  HIP_TRY(hipMemcpy(m->prm, params_host, kNParam * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(m->m1, 0, kNParam * sizeof(float)));
+  HIP_TRY(hipMemset(m->m2, 0, kNParam * sizeof(float)));
+  hipLaunchKernelGGL(k_mlp_sync_w2, dim3(kH * kH / 256), dim3(256), 0, 0, m->prm, m->W2b, m->W2Tb);
+  LAUNCH_CHECK();
+  HIP_TRY(hipDeviceSynchronize());
+  *out = reinterpret_cast<accu_mlp*>(m);
+  return ACCU_OK;
+}
+
+int accu_mlp_destroy(accu_mlp* h) {
+  if (h) {
+    MLP* m = reinterpret_cast<MLP*>(h);
+    m->release();
+    delete m;
+  }
+  return ACCU_OK;
+}
+
+int accu_mlp_params(accu_mlp* h, float* get_host, const float* set_host) {
+  ARG_CHECK(h, "null handle");
+  MLP& m = *reinterpret_cast<MLP*>(h);
+  if (set_host) {
+    HIP_TRY(hipMemcpy(m.prm, set_host, kNParam * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_sync_w2, dim3(kH * kH / 256), dim3(256), 0, 0, m.prm, m.W2b, m.W2Tb);
+    LAUNCH_CHECK();
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  if (get_host) HIP_TRY(hipMemcpy(get_host, m.prm, kNParam * sizeof(float), hipMemcpyDeviceToHost));
+  return ACCU_OK;
+}
+
+int accu_mlp_configure(accu_mlp* h, double rc, double r_on, double r_off, double n_scale, double F0) {
+  ARG_CHECK(h, "null handle");
+  ARG_CHECK(rc > 0 && r_off > r_on && n_scale > 0 && F0 > 0, "bad MLP geometry");
+  MLP& m = *reinterpret_cast<MLP*>(h);
+  m.g.rc = (float)rc;
+  m.g.inv_rc = (float)(1.0 / rc);
+  m.g.rc2 = (float)(rc * rc);
+  m.g.ns = (float)(1.0 / n_scale);
+  m.g.F0 = (float)F0;
+  m.g.r_on = (float)r_on;
+  m.g.r_off = (float)r_off;
+  return ACCU_OK;
+}
+
+// Per-atom inputs for a snapshot: charges (device float64, multiplied by qscale) and
+// coordination numbers from the list (which must reach at least r_off).
+int accu_mlp_prepare(accu_mlp* h, accu_nlist* nlh, const double* pos, const double* q, double qscale, int n,
+                     const double* box, int periodic, void* stream) {
+  ARG_CHECK(h && nlh && pos && q, "null argument");
+  MLP& m = *reinterpret_cast<MLP*>(h);
+  NList& nl = *reinterpret_cast<NList*>(nlh);
+  ARG_CHECK(nl.n == n, "neighbour list built for %d atoms, got %d", nl.n, n);
+  hipStream_t st = (hipStream_t)stream;
+  int rc = m.ensure(n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_to_float, dim3((n + 255) / 256), dim3(256), 0, st, q, qscale, n, m.q);
+  hipLaunchKernelGGL(k_mlp_coord, dim3((n + 255) / 256), dim3(256), 0, st, pos, 3, n, nl.sorted, nl.nn, nl.nbr,
+                     nl.stride, box_of(box, periodic), m.g, m.coord);
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+static int mlp_forward(MLP& m, NList& nl, const double* pos, int n, const Box& b, double* F, int accumulate,
+                       hipStream_t st) {
+  const int blocks = (n + 3) / 4 < 2048 ? (n + 3) / 4 : 2048;
+  hipLaunchKernelGGL(k_mlp_force, dim3(blocks), dim3(256), 0, st, pos, 3, n, nl.sorted, nl.nn, nl.nbr, nl.stride, b,
+                     m.g, m.prm, m.W2b, m.q, m.coord, F, accumulate);
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+int accu_mlp_forces(accu_mlp* h, accu_nlist* nlh, const double* pos, int n, const double* box, int periodic,
+                    int accumulate, double* F, void* stream) {
+  ARG_CHECK(h && nlh && pos && F, "null argument");
+  MLP& m = *reinterpret_cast<MLP*>(h);
+  NList& nl = *reinterpret_cast<NList*>(nlh);
+  ARG_CHECK(nl.n == n && m.cap_n >= n, "call accu_mlp_prepare for this snapshot first");
+  return mlp_forward(m, nl, pos, n, box_of(box, periodic), F, accumulate, (hipStream_t)stream);
+}
+
+// Gradient of the snapshot loss against target (device float64 [n][3], newtons).
+// grad_host (may be NULL) receives the gradient; loss_host the loss (synchronises).
+int accu_mlp_grad(accu_mlp* h, accu_nlist* nlh, const double* pos, int n, const double* box, int periodic,
+                  const double* target, float* grad_host, double* loss_host, void* stream) {
+  ARG_CHECK(h && nlh && pos && target, "null argument");
+  MLP& m = *reinterpret_cast<MLP*>(h);
+  NList& nl = *reinterpret_cast<NList*>(nlh);
+  ARG_CHECK(nl.n == n && m.cap_n >= n, "call accu_mlp_prepare for this snapshot first");
+  hipStream_t st = (hipStream_t)stream;
+  const Box b = box_of(box, periodic);
+  int rc = mlp_forward(m, nl, pos, n, b, m.Fcorr, 0, st);
+  if (rc) return rc;
+  const int blocks = (n + 3) / 4 < 1024 ? (n + 3) / 4 : 1024;
+  const int nw = blocks * 4;
+  if (nw > m.nwaves) {
+    if (m.gpart) HIP_TRY(hipFree(m.gpart));
+    if (m.lpart) HIP_TRY(hipFree(m.lpart));
+    HIP_TRY(hipMalloc(&m.gpart, (size_t)nw * kNParam * sizeof(float)));
+    HIP_TRY(hipMalloc(&m.lpart, (size_t)nw * sizeof(double)));
+    m.nwaves = nw;
+  }
+  hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, st, pos, 3, n, nl.sorted, nl.nn, nl.nbr, nl.stride, b,
+                     m.g, m.prm, m.W2b, m.W2Tb, m.q, m.coord, m.Fcorr, target, 1.0 / (3.0 * n), m.gpart, m.lpart);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, st, m.gpart, m.lpart, nw, m.grad,
+                     m.loss);
+  LAUNCH_CHECK();
+  if (grad_host) HIP_TRY(hipMemcpyAsync(grad_host, m.grad, kNParam * sizeof(float), hipMemcpyDeviceToHost, st));
+  if (loss_host) HIP_TRY(hipMemcpyAsync(loss_host, m.loss, sizeof(double), hipMemcpyDeviceToHost, st));
+  if (grad_host || loss_host) HIP_TRY(hipStreamSynchronize(st));
+  return ACCU_OK;
+}
+
+// One Adam step with the gradient of the last accu_mlp_grad call.
+int accu_mlp_adam(accu_mlp* h, double lr, double b1, double b2, double eps, void* stream) {
+  ARG_CHECK(h, "null handle");
+  MLP& m = *reinterpret_cast<MLP*>(h);
+  m.step += 1;
+  const float c1 = (float)(1.0 / (1.0 - std::pow(b1, m.step)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow(b2, m.step)));
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kNParam + 255) / 256), dim3(256), 0, (hipStream_t)stream, m.prm, m.m1, m.m2,
+                     m.grad, (float)lr, (float)b1, (float)b2, (float)eps, c1, c2, m.W2b, m.W2Tb);
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+}  // extern "C"

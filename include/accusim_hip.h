@@ -117,6 +117,32 @@ int accu_md_stream(accu_md* h, void** stream_out);
 int accu_md_set_timing(accu_md* h, int enable);
 int accu_md_timing(accu_md* h, double* total_ms_host, long long* launches_host);
 
+/* ---- learned DSF -> Ewald pair correction (no counterpart in the reference; it
+ * corrects DampedShiftedCoulomb, forces.py:391-452, toward Ewald, forces.py:333-388).
+ * A 4 -> 64 -> 64 -> 1 pair MLP; parameters are 4545 float32 values laid out as
+ * W1[64][4], b1[64], W2[64][64], b2[64], w3[64], b3. Layer 2 and the backward GEMMs
+ * run on bf16 MFMA with float32 accumulation. */
+typedef struct accu_mlp accu_mlp;
+int accu_mlp_create(accu_mlp** out, const float* params_host);
+int accu_mlp_destroy(accu_mlp* h);
+/* copy the parameters out (get_host) and/or in (set_host); either may be NULL */
+int accu_mlp_params(accu_mlp* h, float* get_host, const float* set_host);
+/* pair cutoff rc, coordination switch r_on..r_off, coordination scale, force scale F0 (SI) */
+int accu_mlp_configure(accu_mlp* h, double rc, double r_on, double r_off, double n_scale, double F0);
+/* per-snapshot inputs: charges (device float64 times qscale) and coordination numbers
+ * from the list h_nl (built with radius >= max(rc, r_off)) */
+int accu_mlp_prepare(accu_mlp* h, accu_nlist* h_nl, const double* pos, const double* q, double qscale, int n,
+                     const double* box_host, int periodic, void* stream);
+/* correction forces (newtons) written to F, or added to it when accumulate != 0 */
+int accu_mlp_forces(accu_mlp* h, accu_nlist* h_nl, const double* pos, int n, const double* box_host, int periodic,
+                    int accumulate, double* F, void* stream);
+/* gradient of the loss mean(((F_corr - target) / F0)^2) for one snapshot; target device
+ * float64 [n][3]; grad_host [4545] and loss_host may be NULL (else this synchronises) */
+int accu_mlp_grad(accu_mlp* h, accu_nlist* h_nl, const double* pos, int n, const double* box_host, int periodic,
+                  const double* target, float* grad_host, double* loss_host, void* stream);
+/* one Adam step with the last gradient */
+int accu_mlp_adam(accu_mlp* h, double lr, double b1, double b2, double eps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
