@@ -9,6 +9,7 @@ from .engine import MDEngine
 from .forces import (DampedShiftedCoulomb, Ewald, Force, Gravity, LennardJones, NeighbourList, VerletList,
                      cell_list_pairs, kvector_columns)
 from .integrators import Integrator, VelocityVerlet, Yoshida4
+from .learned import LearnedDSF, PairCorrection, train_correction
 from .simulation import Simulation, Trajectory
 from .system import Species, System
 from .tree import BarnesHut, direct_subset
@@ -16,7 +17,8 @@ from .units import G, k_B, k_e, si
 
 __all__ = [
     "AccuracyReport", "BarnesHut", "Check", "DampedShiftedCoulomb", "Ewald", "Force", "G", "Gravity", "Integrator",
-    "LennardJones", "MDEngine", "NeighbourList", "Simulation", "Species", "System", "Trajectory", "VelocityVerlet",
+    "LearnedDSF", "LennardJones", "MDEngine", "NeighbourList", "PairCorrection", "Simulation", "Species", "System",
+    "Trajectory", "VelocityVerlet", "train_correction",
     "VerletList", "Yoshida4", "cell_list_pairs", "direct_subset", "k_B", "k_e", "kvector_columns", "si",
     "timestep_convergence",
 ]

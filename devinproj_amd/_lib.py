@@ -154,4 +154,12 @@ SIGNATURES.update({
     "accu_md_stream": [P, P],
     "accu_md_set_timing": [P, I],
     "accu_md_timing": [P, P, P],
+    "accu_mlp_create": [P, P],
+    "accu_mlp_destroy": [P],
+    "accu_mlp_params": [P, P, P],
+    "accu_mlp_configure": [P, D, D, D, D, D],
+    "accu_mlp_prepare": [P, P, P, P, D, I, P, I, P],
+    "accu_mlp_forces": [P, P, P, I, P, I, I, P, P],
+    "accu_mlp_grad": [P, P, P, I, P, I, P, P, P, P],
+    "accu_mlp_adam": [P, D, D, D, D, P],
 })

@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(256) k_mlp_force(const double* __restrict__ po
 // ---------------------------------------------------------------- training
 
 // Per-wave LDS staging for the layout changes between the MFMA products (bf16).
-struct GradScratch {
+// 16-byte aligned: the MFMA operands are read back as 16-byte bf16x8 vectors.
+struct __attribute__((aligned(16))) GradScratch {
   __bf16 h1T[kH * 32];   // [v][p]
   __bf16 dz2[kH * 32];   // [u][p]
   __bf16 dz2T[32 * kH];  // [p][u]
