@@ -163,3 +163,17 @@ SIGNATURES.update({
     "accu_mlp_grad": [P, P, P, I, P, I, P, P, P, P],
     "accu_mlp_adam": [P, D, D, D, D, P],
 })
+
+SIGNATURES.update({
+    "accu_bmd_create": [P, I, P, P, P, P, P, I],
+    "accu_bmd_destroy": [P],
+    "accu_bmd_set_system": [P, I, I, P, P, D, D, D, P],
+    "accu_bmd_set_lj": [P, I, I, P, P, P, D],
+    "accu_bmd_set_dsf": [P, I, D, D, D, D],
+    "accu_bmd_init": [P, D, I],
+    "accu_bmd_run": [P, I, I, P, P, P, I, I],
+    "accu_bmd_sync": [P],
+    "accu_bmd_get": [P, P, P, P, P, I, P, P],
+    "accu_bmd_set_timing": [P, I],
+    "accu_bmd_timing": [P, P, P],
+})

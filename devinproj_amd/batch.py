@@ -1,0 +1,198 @@
+"""Brick-tiled, LDS-staged MD engine for one or many periodic systems.
+
+`BrickEngine([systems], [forces per system], integrator, dts)` advances several
+independent systems in one device arena (the C ABI `accu_bmd_*`). Typical uses:
+  - one large system (fast path for the 1M-atom LJ workload);
+  - the dt, dt/2, dt/4 re-runs of accusim's timestep_convergence
+    (accuracy.py:191-221) for many systems at once: `richardson()` orders the
+    trajectories so that the ones advanced at each tick form a prefix, runs them in
+    one schedule (4 ticks per coarse step: dt/4 runs every tick, dt/2 every other
+    tick, dt once), optionally as replays of one captured hipGraph, and evaluates
+    the reference's Richardson formula on the final positions.
+Brick shapes (the LDS tile) are per system; `autotune.choose_bricks` picks them by
+timing candidates.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from .accuracy import richardson_check
+from .forces import DampedShiftedCoulomb, LennardJones, _erfc
+from .integrators import VelocityVerlet, Yoshida4
+
+LOG_W = 8
+
+
+def default_bricks(s, rc_list, cap_atoms=1500):
+    """Largest brick (cells per axis <= 3) whose staged region stays near cap_atoms."""
+    nc = np.floor(np.asarray(s.box, float) / rc_list).astype(int)
+    per_cell = s.n / float(np.prod(nc))
+    best = (1, 1, 1)
+    for b in [(1, 1, 1), (1, 1, 2), (1, 2, 2), (2, 2, 2), (2, 2, 3), (2, 3, 3), (3, 3, 3)]:
+        if any(b[d] > nc[d] for d in range(3)):
+            continue
+        region = np.prod(np.asarray(b) + 2) * per_cell
+        if region <= cap_atoms:
+            best = b
+    return best
+
+
+class BrickEngine:
+    def __init__(self, systems, forces, integrator=None, dts=None, precision="fp64", bricks=None, slack=0.4,
+                 log_cap=20_000):
+        self.systems = list(systems)
+        S = len(self.systems)
+        if len(forces) != S:
+            raise ValueError("one force list per system")
+        if not isinstance(dts, (list, tuple, np.ndarray)):
+            dts = [dts] * S
+        self.dts = [float(d) for d in dts]
+        self.integrator = integrator or VelocityVerlet()
+        if isinstance(self.integrator, Yoshida4):
+            w = Yoshida4.weights()
+            self.ticks = (3, [S, S, S], [w[0], w[1], w[2]], [w[1], w[2], w[0]])
+        elif isinstance(self.integrator, VelocityVerlet):
+            self.ticks = (1, [S], [1.0], [1.0])
+        else:
+            raise ValueError("the brick engine runs VelocityVerlet or Yoshida4")
+        self.precision = precision
+        L.lib()
+        for s in self.systems:
+            if s.dim != 3 or not s.periodic:
+                raise ValueError("the brick engine needs 3D periodic boxes")
+        natoms = np.array([s.n for s in self.systems], np.int32)
+        pos = np.ascontiguousarray(np.concatenate([s.pos for s in self.systems]), float)
+        vel = np.ascontiguousarray(np.concatenate([s.vel for s in self.systems]), float)
+        sp = np.ascontiguousarray(np.concatenate([s.sp for s in self.systems]), np.int32)
+        box = np.ascontiguousarray(np.stack([np.asarray(s.box, float) for s in self.systems]))
+        h = L.C.c_void_p()
+        L.call("accu_bmd_create", L.C.byref(h), S, L.ptr(natoms), L.ptr(pos), L.ptr(vel), L.ptr(sp), L.ptr(box),
+               L.PRECISIONS[precision])
+        self._h = h
+        self.bricks = []
+        for k, (s, fl) in enumerate(zip(self.systems, forces)):
+            lj = [f for f in fl if isinstance(f, LennardJones)]
+            dsf = [f for f in fl if isinstance(f, DampedShiftedCoulomb)]
+            if len(lj) > 1 or len(dsf) > 1 or len(lj) + len(dsf) != len(fl) or not fl:
+                raise ValueError("each system takes one LennardJones and/or one DampedShiftedCoulomb")
+            radii, skins = [], []
+            mass = np.ascontiguousarray([x.mass for x in s.species], float)
+            charge = np.ascontiguousarray([x.charge for x in s.species], float)
+            if lj:
+                rc, sk = lj[0].list_radius(s)
+                radii.append(rc)
+                skins.append(sk)
+            if dsf:
+                if dsf[0].exclude:
+                    raise ValueError("exclusions are not supported by the device engines")
+                radii.append(dsf[0].rcut)
+                skins.append(dsf[0].skin)
+            skin = min(skins)
+            rc_list = max(r + skin for r in radii)
+            b = tuple(bricks[k]) if bricks is not None else default_bricks(s, rc_list)
+            self.bricks.append(b)
+            bd = np.ascontiguousarray(b, np.int32)
+            L.call("accu_bmd_set_system", h, k, len(s.species), L.ptr(mass), L.ptr(charge), self.dts[k],
+                   float(rc_list), float(skin), L.ptr(bd))
+            if lj:
+                eps, sig2, rc2, shift = lj[0].device_tables(s)
+                L.call("accu_bmd_set_lj", h, k, len(s.species), L.ptr(eps), L.ptr(sig2), L.ptr(rc2), shift)
+            if dsf:
+                f = dsf[0]
+                erc = float(_erfc(np.array(f.alpha * f.rcut)))
+                L.call("accu_bmd_set_dsf", h, k, float(f.alpha), float(f.rcut), erc, float(f.k))
+        self.log_cap = log_cap
+        L.call("accu_bmd_init", h, float(slack), int(log_cap))
+        self.steps = 0
+
+    def run(self, steps, record_every=1, use_graph=False, sync=True, schedule=None):
+        nt, active, wc, wn = schedule or self.ticks
+        a = np.ascontiguousarray(active, np.int32)
+        c = np.ascontiguousarray(wc, float)
+        n = np.ascontiguousarray(wn, float)
+        L.call("accu_bmd_run", self._h, int(steps), int(nt), L.ptr(a), L.ptr(c), L.ptr(n), int(record_every),
+               int(bool(use_graph)))
+        self.steps += steps
+        if sync:
+            self.sync()
+        return self
+
+    def sync(self):
+        L.call("accu_bmd_sync", self._h)
+
+    def fetch(self, state=True):
+        S = len(self.systems)
+        N = sum(s.n for s in self.systems)
+        pos = np.empty((N, 3)) if state else None
+        vel = np.empty((N, 3)) if state else None
+        F = np.empty((N, 3)) if state else None
+        log = np.zeros((S, self.log_cap, LOG_W))
+        stats = np.zeros((S, 4), np.int64)
+        info = np.zeros(4, np.int64)
+        L.call("accu_bmd_get", self._h, L.ptr(pos), L.ptr(vel), L.ptr(F), L.ptr(log), self.log_cap, L.ptr(stats),
+               L.ptr(info))
+        self.info = {"max_neighbours": int(info[0]), "capacity": int(info[1]), "region_capacity": int(info[2])}
+        self.rebuilds = stats[:, 0].copy()
+        self.logs = [log[k, :int(stats[k, 1])].copy() for k in range(S)]
+        if state:
+            off = 0
+            self.forces_out = []
+            for s in self.systems:
+                s.pos[...] = pos[off:off + s.n]
+                s.vel[...] = vel[off:off + s.n]
+                self.forces_out.append(F[off:off + s.n])
+                off += s.n
+        return self.logs
+
+    def energy_drift(self, k=0):
+        lg = self.logs[k]
+        E = lg[:, 0] + lg[:, 1]
+        scale = max(abs(E[0]), np.max(np.abs(lg[:, 1])), np.max(np.abs(lg[:, 0])), 1e-300)
+        return float(np.max(np.abs(E - E[0])) / scale)
+
+    def set_timing(self, enable=True):
+        L.call("accu_bmd_set_timing", self._h, int(bool(enable)))
+
+    def timing(self):
+        ms, k = L.C.c_double(0.0), L.C.c_longlong(0)
+        L.call("accu_bmd_timing", self._h, L.C.byref(ms), L.C.byref(k))
+        return ms.value, k.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and L._lib is not None:
+            L._lib.accu_bmd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def richardson(systems, make_forces, dt, duration_steps, precision="mixed", bricks=None, use_graph=True, block=10,
+               log=None):
+    """timestep_convergence (accuracy.py:191-221) for many systems in one batched run.
+
+    Every system is re-run from its current state at dt, dt/2 and dt/4 for
+    `duration_steps` coarse steps (= duration_steps * k steps at dt/k); returns one
+    Check per system and the engine (for timing)."""
+    S = len(systems)
+    # order: all dt/4 replicas, then dt/2, then dt, so each tick advances a prefix
+    order = [(k, 4) for k in range(S)] + [(k, 2) for k in range(S)] + [(k, 1) for k in range(S)]
+    reps = [systems[k].copy() for k, _ in order]
+    dts = [dt / m for _, m in order]
+    forces = [make_forces(r) for r in reps]
+    br = None if bricks is None else [bricks[k] for k, _ in order]
+    eng = BrickEngine(reps, forces, VelocityVerlet(), dts, precision=precision, bricks=br)
+    sched = (4, [S, S, 2 * S, 3 * S], [1.0] * 4, [1.0] * 4)
+    left = duration_steps
+    while left > 0:
+        blk = min(block, left) if use_graph else left
+        eng.run(blk, record_every=10 ** 9, use_graph=use_graph and blk == block, sync=False, schedule=sched)
+        left -= blk
+    eng.sync()
+    eng.fetch(state=True)
+    checks = []
+    for k in range(S):
+        finals = [reps[k + j * S].pos.copy() for j in (2, 1, 0)]  # dt, dt/2, dt/4
+        checks.append(richardson_check(finals, duration_steps, 2))
+    return checks, eng, reps

@@ -1,0 +1,1227 @@
+// Brick-tiled, LDS-staged MD engine for one or many periodic systems.
+//
+// Same physics as the single-system engine (integrators.py:19-47 with the Verlet rule
+// of forces.py:86-98 over LJ forces.py:119-142 and DSF forces.py:408-438), laid out for
+// the LDS: cells of each system are grouped into small bricks (bx x by x bz cells),
+// cell ids are numbered brick by brick, and atoms are kept sorted by cell id, so a
+// brick's home atoms are one contiguous range. A workgroup owns one brick:
+//   1. it stages the atoms of the (bx+2)(by+2)(bz+2) region cells around the brick
+//      into LDS, shifted by their periodic image and taken relative to the brick
+//      centre (float64, or float32 in mixed mode), and
+//   2. its home atoms read neighbours from LDS through a Verlet list of 16-bit local
+//      indices into that region (built the same way at every rebuild).
+// Neighbour positions therefore come from LDS instead of L2/HBM gathers, the list is
+// half the bytes, and no minimum-image arithmetic is left in the pair loop.
+//
+// Several systems (e.g. Richardson replicas at dt, dt/2, dt/4 of many sizes) share one
+// arena: systems are ordered so that the systems advanced at each "tick" form a prefix
+// of the brick array, each system carries its own box, interactions, dt and rebuild
+// flag, and every kernel of the rebuild chain skips systems whose flag is clear.
+#include "common.h"
+#include "pairkern.h"
+#include "../../include/accusim_hip.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace accu {
+
+constexpr int kBT = 256;
+constexpr int kMaxRegionCells = 256;
+constexpr int kRecW = 8;  // PE, KE, Px, Py, Pz, rebuilds, 0, 0
+
+struct BSys {
+  double L[3], inv[3];
+  double cinv[3];  // ncell / L
+  int nc[3];
+  int cell_off, ncells;
+  int atom_off, natoms;
+  int brick_off, nbricks;
+  int cellmap_off;
+  int ns;
+  double mass[kMaxSpecies], charge[kMaxSpecies];
+  int has_lj, has_coul;
+  double lj_eps[kMaxSpecies * kMaxSpecies], lj_sig2[kMaxSpecies * kMaxSpecies], lj_rc2[kMaxSpecies * kMaxSpecies];
+  double lj_shift;
+  CoulParams coul;
+  double rc_list2;
+  double half_skin2;
+  double dt;
+};
+
+struct BBrick {
+  int sys;
+  int cell0, nhome;  // home cells: global ids [cell0, cell0 + nhome)
+  int hd[3], rd[3];
+  int reg_off;
+  double org[3];
+};
+
+struct BRegion {
+  int gid;
+  int shift;  // (sx+1) + 3 (sy+1) + 9 (sz+1), image offset s * L
+};
+
+template <typename P>
+struct P4 {};
+template <>
+struct P4<double> {
+  typedef double4 T;
+};
+template <>
+struct P4<float> {
+  typedef float4 T;
+};
+
+// ------------------------------------------------------------------ region staging
+
+// Region cell starts (exclusive prefix of counts) into rs[0..R]; returns the total.
+__device__ int region_starts(const BBrick& b, const BRegion* __restrict__ reg, const int* __restrict__ start, int* rs) {
+  const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  for (int r = threadIdx.x; r < R; r += blockDim.x) {
+    const int c = reg[b.reg_off + r].gid;
+    rs[r + 1] = start[c + 1] - start[c];
+  }
+  if (threadIdx.x == 0) rs[0] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int r = 1; r <= R; ++r) rs[r] += rs[r - 1];
+  __syncthreads();
+  return rs[R];
+}
+
+// Stage region atoms: lds[q] = pos + shift * L - org, species in w.
+template <typename P>
+__device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
+                             const int* __restrict__ start, const int* rs, int total, const double4* __restrict__ pos,
+                             typename P4<P>::T* lds) {
+  const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    int lo = 0, hi = R - 1;  // region cell r with rs[r] <= q < rs[r+1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rs[mid] <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const BRegion rc = reg[b.reg_off + lo];
+    const int a = start[rc.gid] + (q - rs[lo]);
+    const double4 x = pos[a];
+    const int sx = rc.shift % 3 - 1, sy = (rc.shift / 3) % 3 - 1, sz = rc.shift / 9 - 1;
+    const double X = (x.x + sx * s.L[0]) - b.org[0];
+    const double Y = (x.y + sy * s.L[1]) - b.org[1];
+    const double Z = (x.z + sz * s.L[2]) - b.org[2];
+    typename P4<P>::T v;
+    v.x = (P)X;
+    v.y = (P)Y;
+    v.z = (P)Z;
+    v.w = (P)x.w;
+    lds[q] = v;
+  }
+}
+
+__device__ __forceinline__ int home_region(const BBrick& b, int local) {
+  // home cell local index (z fastest) -> region cell of the same cell
+  const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
+  return ((lx + 1) * b.rd[1] + (ly + 1)) * b.rd[2] + (lz + 1);
+}
+
+// ------------------------------------------------------------------ rebuild chain
+
+__global__ void k_b_zero(const int* __restrict__ cellsys, int* __restrict__ count, int ncells,
+                         const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells && flags[cellsys[c]]) count[c] = 0;
+}
+
+// Each system's current positions live in posA or posB according to par[system]
+// (double buffering per system, so systems can advance at different rates).
+__device__ __forceinline__ const double4* cur_buf(const double4* A, const double4* B, const int* par, int s) {
+  return par[s] ? B : A;
+}
+
+__global__ void k_b_assign(const double4* __restrict__ posA, const double4* __restrict__ posB,
+                           const int* __restrict__ par, const int* __restrict__ sysid, const BSys* __restrict__ sys,
+                           const int* __restrict__ cellmap, int* __restrict__ cell, int* __restrict__ count, int natoms,
+                           const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natoms) return;
+  const int si = sysid[a];
+  if (!flags[si]) return;
+  const BSys& s = sys[si];
+  const double4 x = cur_buf(posA, posB, par, si)[a];
+  const double xs[3] = {x.x, x.y, x.z};
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double t = xs[d] * s.cinv[d];
+    const int m = s.nc[d] - 1;
+    c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : (int)t);
+  }
+  const int gid = cellmap[s.cellmap_off + (c[0] * s.nc[1] + c[1]) * s.nc[2] + c[2]];
+  cell[a] = gid;
+  atomicAdd(&count[gid], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_b_scan(const int* __restrict__ in, int* __restrict__ out,
+                                                 int* __restrict__ out2, int n, const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  __shared__ long long s[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int a = t * per, b = (a + per < n) ? a + per : n;
+  long long sum = 0;
+  for (int i = a; i < b; ++i) sum += in[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    long long v = (t >= o) ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  long long run = s[t] - sum;
+  for (int i = a; i < b; ++i) {
+    const int v = in[i];
+    out[i] = (int)run;
+    out2[i] = (int)run;
+    run += v;
+  }
+  if (t == 1023) {
+    out[n] = (int)s[1023];
+    out2[n] = (int)s[1023];
+  }
+}
+
+__global__ void k_b_scatter(const int* __restrict__ cell, const int* __restrict__ sysid, int natoms,
+                            int* __restrict__ cursor, int* __restrict__ tmp, const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natoms || !flags[sysid[a]]) return;
+  tmp[atomicAdd(&cursor[cell[a]], 1)] = a;
+}
+
+__global__ void k_b_sort(const int* __restrict__ start, const int* __restrict__ cellsys, int ncells,
+                         const int* __restrict__ tmp, int* __restrict__ sorted, const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= ncells || !flags[cellsys[c]]) return;
+  const int lane = threadIdx.x & 63;
+  const int a = start[c], len = start[c + 1] - a;
+  for (int e = lane; e < len; e += 64) {
+    const int v = tmp[a + e];
+    int r = 0;
+    for (int x = 0; x < len; ++x) r += (tmp[a + x] < v);
+    sorted[a + r] = v;
+  }
+}
+
+__global__ void k_b_permute(int natoms, const int* __restrict__ sysid, const int* __restrict__ sorted,
+                            const double4* __restrict__ posA, const double4* __restrict__ posB,
+                            const int* __restrict__ par, const double4* __restrict__ vel,
+                            const double4* __restrict__ F, const int* __restrict__ perm, const int* __restrict__ cell,
+                            double4* __restrict__ pos2, double4* __restrict__ vel2, double4* __restrict__ F2,
+                            int* __restrict__ perm2, int* __restrict__ cell2, const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms || !flags[sysid[t]]) return;
+  const int i = sorted[t];
+  pos2[t] = cur_buf(posA, posB, par, sysid[t])[i];
+  vel2[t] = vel[i];
+  F2[t] = F[i];
+  perm2[t] = perm[i];
+  cell2[t] = cell[i];
+}
+
+__global__ void k_b_copyback(int natoms, const int* __restrict__ sysid, double4* __restrict__ posA,
+                             double4* __restrict__ posB, const int* __restrict__ par, double4* __restrict__ ref, double4* __restrict__ vel, double4* __restrict__ F,
+                             int* __restrict__ perm, int* __restrict__ cell, const double4* __restrict__ pos2,
+                             const double4* __restrict__ vel2, const double4* __restrict__ F2,
+                             const int* __restrict__ perm2, const int* __restrict__ cell2,
+                             const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms || !flags[sysid[t]]) return;
+  (par[sysid[t]] ? posB : posA)[t] = pos2[t];
+  ref[t] = pos2[t];
+  vel[t] = vel2[t];
+  F[t] = F2[t];
+  perm[t] = perm2[t];
+  cell[t] = cell2[t];
+}
+
+// Verlet list of one brick: every home atom against the 27 region cells around its
+// own cell, float64 distance test |d|^2 < rc_list^2, 16-bit local indices.
+__global__ void __launch_bounds__(kBT) k_b_build(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
+                                                 const BRegion* __restrict__ reg, const int* __restrict__ start,
+                                                 const double4* __restrict__ posA, const double4* __restrict__ posB,
+                                                 const int* __restrict__ par, const int* __restrict__ cell,
+                                                 int* __restrict__ nn, unsigned short* __restrict__ nbr, int stride,
+                                                 int max_nbrs, int cap, int* __restrict__ maxcount,
+                                                 int* __restrict__ overflow, const int* __restrict__ flags, int S) {
+  if (flags[S] == 0) return;
+  const BBrick b = bricks[blockIdx.x];
+  if (!flags[b.sys]) return;
+  const BSys& s = sys[b.sys];
+  __shared__ int rs[kMaxRegionCells + 1];
+  extern __shared__ __align__(16) double4 lds64[];
+  const int total = region_starts(b, reg, start, rs);
+  if (total > cap) {
+    if (threadIdx.x == 0) atomicMax(overflow, total);
+    return;
+  }
+  stage_region<double>(b, s, reg, start, rs, total, cur_buf(posA, posB, par, b.sys), lds64);
+  __syncthreads();
+  const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
+  int mymax = 0;
+  for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+    const int local = cell[a] - b.cell0;
+    const int rh = home_region(b, local);
+    const int self = rs[rh] + (a - start[b.cell0 + local]);
+    const double4 xi = lds64[self];
+    const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
+    int cnt = 0;
+    for (int o = 0; o < 27; ++o) {
+      const int r = ((lx + o / 9) * b.rd[1] + (ly + (o / 3) % 3)) * b.rd[2] + (lz + o % 3);
+      for (int q = rs[r]; q < rs[r + 1]; ++q) {
+        if (q == self) continue;
+        const double4 xj = lds64[q];
+        const double dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        if (r2_np(dx, dy, dz) < s.rc_list2) {
+          if (cnt < max_nbrs) nbr[(size_t)cnt * stride + a] = (unsigned short)q;
+          ++cnt;
+        }
+      }
+    }
+    nn[a] = cnt < max_nbrs ? cnt : max_nbrs;
+    mymax = cnt > mymax ? cnt : mymax;
+  }
+  // one atomic per wave
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxcount, mymax);
+}
+
+__global__ void k_b_end(int* __restrict__ flags, int S, int* __restrict__ nreb) {
+  if (flags[S] == 0) return;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    if (flags[s]) {
+      nreb[s] += 1;
+      flags[s] = 0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) flags[S] = 0;
+}
+
+// ------------------------------------------------------------------ fused step
+
+__device__ __forceinline__ double np_mod2(double x, double L) {
+  double r = fmod(x, L);
+  if (r != 0.0) {
+    if ((r < 0.0) != (L < 0.0)) r += L;
+  } else {
+    r = copysign(0.0, L);
+  }
+  return r;
+}
+
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
+__device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P qi, const typename P4<P>::T& xi,
+                                       const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& cx, P& cy, P& cz,
+                                       P& cu) {
+  const P dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+  const int tj = ONE_SP ? 0 : (int)xj.w;
+  if constexpr (sizeof(P) == 8) {
+    const double r2 = r2_np(dx, dy, dz);
+    if constexpr (LJ_ON) {
+      const int pj = row + tj;
+      lj_pair64(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192], fx, fy, fz, u);
+    }
+    if constexpr (COUL_ON) coul_pair64(s.coul, dx, dy, dz, r2, qi * s.charge[tj], cx, cy, cz, cu);
+  } else {
+    const float r2 = r2_npf(dx, dy, dz);
+    if constexpr (LJ_ON) {
+      const int pj = row + tj;
+      lj_pair32(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192], fx, fy, fz, u);
+    }
+    if constexpr (COUL_ON)
+      coul_pair32((float)s.coul.alpha, (float)s.coul.rc, (float)s.coul.c, (float)s.coul.erc_rc, (float)s.coul.fshift,
+                  s.coul.shifted, dx, dy, dz, r2, qi * (float)s.charge[tj], cx, cy, cz, cu);
+  }
+}
+
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
+__global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
+                                                 const BRegion* __restrict__ reg, const int* __restrict__ start,
+                                                 double4* __restrict__ posA, double4* __restrict__ posB,
+                                                 const int* __restrict__ par, double4* __restrict__ vel,
+                                                 double4* __restrict__ Fst,
+                                                 const double4* __restrict__ ref, const int* __restrict__ cell,
+                                                 const int* __restrict__ nn, const unsigned short* __restrict__ nbr,
+                                                 int stride, int cap, double wc, double wn, int drift_from,
+                                                 int record, int* __restrict__ flags, int S,
+                                                 double* __restrict__ partials, int* __restrict__ overflow) {
+  typedef typename P4<P>::T T4;
+  const BBrick b = bricks[blockIdx.x];
+  const BSys& s = sys[b.sys];
+  __shared__ int rs[kMaxRegionCells + 1];
+  __shared__ P ljt[3 * 64 + 1];
+  __shared__ double red[5 * 4];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T4* lds = reinterpret_cast<T4*>(lds_raw);
+  const double4* __restrict__ pos = par[b.sys] ? posB : posA;
+  double4* __restrict__ pos_next = par[b.sys] ? posA : posB;
+  if constexpr (LJ_ON) {
+    const int m = s.ns * s.ns;
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+      ljt[k] = (P)s.lj_eps[k];
+      ljt[64 + k] = (P)s.lj_sig2[k];
+      ljt[128 + k] = (P)s.lj_rc2[k];
+    }
+    if (threadIdx.x == 0) ljt[192] = (P)s.lj_shift;
+  }
+  const int total = region_starts(b, reg, start, rs);
+  if (total > cap) {  // uniform: the whole block leaves; the host reports the overflow
+    if (threadIdx.x == 0) atomicMax(overflow, total);
+    return;
+  }
+  stage_region<P>(b, s, reg, start, rs, total, pos, lds);
+  __syncthreads();
+  const double h_cur = 0.5 * wc * s.dt, h_next = 0.5 * wn * s.dt, dt_next = wn * s.dt;
+  const bool do_drift = (int)blockIdx.x < drift_from;
+  const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
+  double acc[5] = {0, 0, 0, 0, 0};
+  bool moved_far = false;
+  for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+    const int local = cell[a] - b.cell0;
+    const int self = rs[home_region(b, local)] + (a - start[b.cell0 + local]);
+    const T4 xi = lds[self];
+    const int ti = ONE_SP ? 0 : (int)xi.w;
+    const int row = ti * s.ns;
+    const P qi = (P)s.charge[ti];
+    const int cnt = nn[a];
+    double Fx = 0, Fy = 0, Fz = 0, U = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
+    P fx = 0, fy = 0, fz = 0, u = 0, cx = 0, cy = 0, cz = 0, cu = 0;
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+      const int l0 = nbr[(size_t)k * stride + a], l1 = nbr[(size_t)(k + 1) * stride + a];
+      const int l2 = nbr[(size_t)(k + 2) * stride + a], l3 = nbr[(size_t)(k + 3) * stride + a];
+      const T4 x0 = lds[l0], x1 = lds[l1], x2 = lds[l2], x3 = lds[l3];
+      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, cx, cy, cz, cu);
+      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, cx, cy, cz, cu);
+      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, cx, cy, cz, cu);
+      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, cx, cy, cz, cu);
+      if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 4 neighbours
+        Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+        fx = fy = fz = u = cx = cy = cz = cu = 0;
+      }
+    }
+    for (; k < cnt; ++k)
+      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, lds[nbr[(size_t)k * stride + a]], fx, fy, fz, u, cx, cy,
+                                        cz, cu);
+    Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+    if constexpr (COUL_ON) {
+      const double q = s.charge[ti];
+      CU -= q * q * s.coul.self;
+      Fx += s.coul.k * Cx;
+      Fy += s.coul.k * Cy;
+      Fz += s.coul.k * Cz;
+      U += s.coul.k * CU;
+    }
+    const double m = s.mass[ti];
+    const double ax = Fx / m, ay = Fy / m, az = Fz / m;
+    double4 v = vel[a];
+    {
+#pragma clang fp contract(off)
+      v.x = v.x + h_cur * ax;
+      v.y = v.y + h_cur * ay;
+      v.z = v.z + h_cur * az;
+    }
+    if (record) {
+      acc[0] += U;
+      acc[1] += 0.5 * m * ((v.x * v.x + v.z * v.z) + v.y * v.y);
+      acc[2] += m * v.x;
+      acc[3] += m * v.y;
+      acc[4] += m * v.z;
+    }
+    Fst[a] = make_double4(Fx, Fy, Fz, 0.0);
+    if (do_drift) {
+      double4 x = pos[a];
+      {
+#pragma clang fp contract(off)
+        v.x = v.x + h_next * ax;
+        v.y = v.y + h_next * ay;
+        v.z = v.z + h_next * az;
+        x.x = x.x + dt_next * v.x;
+        x.y = x.y + dt_next * v.y;
+        x.z = x.z + dt_next * v.z;
+      }
+      x.x = np_mod2(x.x, s.L[0]);
+      x.y = np_mod2(x.y, s.L[1]);
+      x.z = np_mod2(x.z, s.L[2]);
+      pos_next[a] = x;
+      const double4 r = ref[a];
+      const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+      const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+      const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+      moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+    }
+    vel[a] = v;
+  }
+  if (__any(moved_far) && (threadIdx.x & 63) == 0) {
+    atomicOr(&flags[b.sys], 1);
+    atomicOr(&flags[S], 1);
+  }
+  if (record) {
+    block_sum<5>(acc, red);
+    if (threadIdx.x == 0)
+      for (int k2 = 0; k2 < 5; ++k2) partials[(size_t)blockIdx.x * 5 + k2] = acc[k2];
+  }
+}
+
+// Opening half kick + drift of the first (sub)step for the atoms of the active systems.
+__global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ sysid, int natoms,
+                          double4* __restrict__ posA, double4* __restrict__ posB, const int* __restrict__ par,
+                          double4* __restrict__ vel, const double4* __restrict__ Fst, const double4* __restrict__ ref,
+                          double wn, int* __restrict__ flags, int S) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  bool moved_far = false;
+  int si = 0;
+  if (a < natoms) {
+    si = sysid[a];
+    const BSys& s = sys[si];
+    const double4* pos = par[si] ? posB : posA;
+    double4* pos_next = par[si] ? posA : posB;
+    const double4 x0 = pos[a];
+    const double m = s.mass[(int)x0.w];
+    const double4 f = Fst[a];
+    const double h = 0.5 * wn * s.dt, dtn = wn * s.dt;
+    double4 v = vel[a], x = x0;
+    {
+#pragma clang fp contract(off)
+      v.x = v.x + h * (f.x / m);
+      v.y = v.y + h * (f.y / m);
+      v.z = v.z + h * (f.z / m);
+      x.x = x.x + dtn * v.x;
+      x.y = x.y + dtn * v.y;
+      x.z = x.z + dtn * v.z;
+    }
+    x.x = np_mod2(x.x, s.L[0]);
+    x.y = np_mod2(x.y, s.L[1]);
+    x.z = np_mod2(x.z, s.L[2]);
+    vel[a] = v;
+    pos_next[a] = x;
+    const double4 r = ref[a];
+    const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+    const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+    const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+    moved_far = !(r2_np(d0, d1, d2) < s.half_skin2);
+    if (moved_far) {
+      atomicOr(&flags[si], 1);
+      atomicOr(&flags[S], 1);
+    }
+  }
+}
+
+// Per-system record: sum the bricks' partials in brick order and append to the log.
+__global__ void k_b_record(const BSys* __restrict__ sys, const double* __restrict__ partials,
+                           double* __restrict__ log, int log_cap, int* __restrict__ log_count,
+                           const int* __restrict__ nreb) {
+  const int si = blockIdx.x;
+  const BSys& s = sys[si];
+  __shared__ double red[5 * 4];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
+    for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
+  block_sum<5>(acc, red);
+  if (threadIdx.x == 0) {
+    const int idx = log_count[si];
+    if (idx < log_cap) {
+      double* o = log + ((size_t)si * log_cap + idx) * kRecW;
+      for (int k = 0; k < 5; ++k) o[k] = acc[k];
+      o[5] = (double)nreb[si];
+      o[6] = 0.0;
+      o[7] = 0.0;
+    }
+    log_count[si] = idx + 1;
+  }
+}
+
+__global__ void k_b_set_all(int* flags, int S) {
+  for (int s = threadIdx.x; s <= S; s += blockDim.x) flags[s] = 1;
+}
+
+__global__ void k_b_flip(int* __restrict__ par, int na) {
+  for (int s = threadIdx.x; s < na; s += blockDim.x) par[s] ^= 1;
+}
+
+// a (or, with b != NULL, the buffer par selects per system) scattered to user order
+__global__ void k_b_scatter_user(int natoms, const int* __restrict__ perm, const double4* __restrict__ a,
+                                 const double4* __restrict__ b, const int* __restrict__ par,
+                                 const int* __restrict__ sysid, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms) return;
+  const int u = perm[t];
+  const double4 v = (b && par[sysid[t]]) ? b[t] : a[t];
+  out[3 * (size_t)u] = v.x;
+  out[3 * (size_t)u + 1] = v.y;
+  out[3 * (size_t)u + 2] = v.z;
+}
+
+// ------------------------------------------------------------------ host engine
+
+struct SysSpec {
+  int natoms = 0;
+  double box[3] = {0, 0, 0};
+  int ns = 1;
+  double mass[kMaxSpecies] = {0}, charge[kMaxSpecies] = {0};
+  bool has_lj = false, has_coul = false;
+  double eps[64], sig2[64], rc2[64], shift = 0;
+  double alpha = 0, rcut = 0, erc = 0, kc = 0;
+  double rc_list = 0, skin = 0;
+  int bdim[3] = {2, 2, 2};
+  double dt = 0;
+};
+
+struct BMD {
+  int S = 0, N = 0, precision = ACCU_FP64;
+  std::vector<SysSpec> spec;
+  hipStream_t st = nullptr;
+  // tables
+  BSys* d_sys = nullptr;
+  BBrick* d_bricks = nullptr;
+  BRegion* d_reg = nullptr;
+  int* d_cellmap = nullptr;
+  int* d_cellsys = nullptr;
+  int* d_sysid = nullptr;
+  int ncells = 0, nbricks = 0;
+  std::vector<int> brick_off, atom_off;
+  // state
+  double4 *pos = nullptr, *pos_b = nullptr, *vel = nullptr, *F = nullptr, *ref = nullptr;  // pos/pos_b: buffers A/B
+  int* par = nullptr;  // per system: 1 when its current positions are in pos_b
+  double4 *pos2 = nullptr, *vel2 = nullptr, *F2 = nullptr;
+  int *perm = nullptr, *perm2 = nullptr, *cell = nullptr, *cell2 = nullptr;
+  int *count = nullptr, *start = nullptr, *cursor = nullptr, *tmp = nullptr, *sorted = nullptr;
+  int* nn = nullptr;
+  unsigned short* nbr = nullptr;
+  int stride = 0, max_nbrs = 0;
+  int* flags = nullptr;  // [S] per system + [S] any
+  int* ints = nullptr;   // [0] max neighbours, [1] region overflow
+  int* nreb = nullptr;
+  int* log_count = nullptr;
+  double* log = nullptr;
+  int log_cap = 0;
+  double* partials = nullptr;
+  int cap_fused = 0, cap_build = 0;  // region capacity (atoms) for the LDS
+  bool ready = false;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> graph_key;
+
+  void release() {
+    void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, vel, F, ref, pos2, vel2, F2,
+                  perm, perm2, cell, cell2, count, start, cursor, tmp, sorted, nn, nbr, flags, ints, nreb, log_count,
+                  log, partials};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  size_t lds_fused() const { return (size_t)cap_fused * (precision == ACCU_FP64 ? sizeof(double4) : sizeof(float4)); }
+  size_t lds_build() const { return (size_t)cap_build * sizeof(double4); }
+
+  void launch_rebuild() {
+    const int T = 256;
+    hipLaunchKernelGGL(k_b_zero, dim3((ncells + T - 1) / T), dim3(T), 0, st, d_cellsys, count, ncells, flags, S);
+    hipLaunchKernelGGL(k_b_assign, dim3((N + T - 1) / T), dim3(T), 0, st, pos, pos_b, par, d_sysid, d_sys, d_cellmap,
+                       cell, count, N, flags, S);
+    hipLaunchKernelGGL(k_b_scan, dim3(1), dim3(1024), 0, st, count, start, cursor, ncells, flags, S);
+    hipLaunchKernelGGL(k_b_scatter, dim3((N + T - 1) / T), dim3(T), 0, st, cell, d_sysid, N, cursor, tmp, flags, S);
+    hipLaunchKernelGGL(k_b_sort, dim3((ncells + 3) / 4), dim3(256), 0, st, start, d_cellsys, ncells, tmp, sorted, flags,
+                       S);
+    hipLaunchKernelGGL(k_b_permute, dim3((N + T - 1) / T), dim3(T), 0, st, N, d_sysid, sorted, pos, pos_b, par, vel, F,
+                       perm, cell, pos2, vel2, F2, perm2, cell2, flags, S);
+    hipLaunchKernelGGL(k_b_copyback, dim3((N + T - 1) / T), dim3(T), 0, st, N, d_sysid, pos, pos_b, par, ref, vel, F,
+                       perm, cell, pos2, vel2, F2, perm2, cell2, flags, S);
+    hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(kBT), lds_build(), st, d_sys, d_bricks, d_reg, start, pos, pos_b,
+                       par, cell, nn, nbr, stride, max_nbrs, cap_build, ints, ints + 1, flags, S);
+    hipLaunchKernelGGL(k_b_end, dim3(1), dim3(256), 0, st, flags, S, nreb);
+  }
+
+  template <typename P, bool ONE>
+  void fused_t(int nb, double wc, double wn, int drift_from, int rec, bool lj, bool co) {
+    const size_t sh = lds_fused();
+    if (lj && co)
+      hipLaunchKernelGGL((k_b_fused<P, true, true, ONE>), dim3(nb), dim3(kBT), sh, st, d_sys, d_bricks, d_reg, start,
+                         pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
+                         flags, S, partials, ints + 1);
+    else if (lj)
+      hipLaunchKernelGGL((k_b_fused<P, true, false, ONE>), dim3(nb), dim3(kBT), sh, st, d_sys, d_bricks, d_reg, start,
+                         pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
+                         flags, S, partials, ints + 1);
+    else
+      hipLaunchKernelGGL((k_b_fused<P, false, true, ONE>), dim3(nb), dim3(kBT), sh, st, d_sys, d_bricks, d_reg, start,
+                         pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
+                         flags, S, partials, ints + 1);
+  }
+
+  bool any_lj = false, any_coul = false, one_sp = true;
+
+  // One fused (sub)step of the first `nb` bricks; bricks below drift_from also drift.
+  void launch_fused(int nb, double wc, double wn, int drift_from, int rec) {
+    if (timing) {
+      if (ev_used + 2 > ev.size()) {
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        ev.push_back(a);
+        ev.push_back(b);
+      }
+      (void)hipEventRecord(ev[ev_used], st);
+    }
+    if (precision == ACCU_FP64) {
+      if (one_sp) fused_t<double, true>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
+      else fused_t<double, false>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
+    } else {
+      if (one_sp) fused_t<float, true>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
+      else fused_t<float, false>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
+    }
+    if (timing) {
+      (void)hipEventRecord(ev[ev_used + 1], st);
+      ev_used += 2;
+    }
+    if (rec) hipLaunchKernelGGL(k_b_record, dim3(S), dim3(256), 0, st, d_sys, partials, log, log_cap, log_count, nreb);
+  }
+
+  // Enqueue `coarse` coarse steps of `nticks` ticks; at tick t the first active[t]
+  // systems advance one (sub)step with weights wc[t] (closing kick) and wn[t] (next
+  // opening kick and drift). The last tick must advance every system. Each system's
+  // buffer parity flips with every drift, on the device.
+  void enqueue(int coarse, int nticks, const int* active, const double* wc, const double* wn, int record_every) {
+    const int T = 256;
+    hipLaunchKernelGGL(k_b_drift, dim3((N + T - 1) / T), dim3(T), 0, st, d_sys, d_sysid, N, pos, pos_b, par, vel, F,
+                       ref, wc[0], flags, S);
+    hipLaunchKernelGGL(k_b_flip, dim3(1), dim3(256), 0, st, par, S);
+    launch_rebuild();
+    for (int c = 0; c < coarse; ++c) {
+      for (int t = 0; t < nticks; ++t) {
+        const int na = active[t];
+        const int nb = brick_off[na];
+        const bool last = (c == coarse - 1) && (t == nticks - 1);
+        const int rec = (t == nticks - 1) && record_every > 0 && ((c + 1) % record_every == 0 || c == coarse - 1);
+        launch_fused(nb, wc[t], wn[t], last ? 0 : nb, rec);
+        if (!last) {
+          hipLaunchKernelGGL(k_b_flip, dim3(1), dim3(256), 0, st, par, na);
+          launch_rebuild();
+        }
+      }
+    }
+  }
+};
+
+static int build_tables(BMD& m) {
+  const int S = m.S;
+  std::vector<BSys> hs(S);
+  std::vector<BBrick> hb;
+  std::vector<BRegion> hr;
+  std::vector<int> cellmap, cellsys;
+  std::vector<std::array<int, 3>> corner;  // low corner (cell coords) of every brick
+  m.brick_off.assign(S + 1, 0);
+  m.atom_off.assign(S + 1, 0);
+  int cell_off = 0;
+  for (int si = 0; si < S; ++si) {
+    const SysSpec& sp = m.spec[si];
+    BSys& s = hs[si];
+    memset(&s, 0, sizeof s);
+    for (int d = 0; d < 3; ++d) {
+      s.L[d] = sp.box[d];
+      s.inv[d] = 1.0 / sp.box[d];
+      s.nc[d] = (int)std::floor(sp.box[d] / sp.rc_list);
+      ARG_CHECK(s.nc[d] >= 3, "system %d: box %.3g is under 3 list radii (%.3g) along axis %d", si, sp.box[d],
+                sp.rc_list, d);
+      ARG_CHECK(sp.bdim[d] >= 1 && sp.bdim[d] <= s.nc[d], "system %d: brick dim %d outside [1, %d]", si, sp.bdim[d],
+                s.nc[d]);
+      s.cinv[d] = s.nc[d] / sp.box[d];
+    }
+    s.cell_off = cell_off;
+    s.ncells = s.nc[0] * s.nc[1] * s.nc[2];
+    s.atom_off = m.atom_off[si];
+    s.natoms = sp.natoms;
+    m.atom_off[si + 1] = m.atom_off[si] + sp.natoms;
+    s.brick_off = (int)hb.size();
+    s.cellmap_off = (int)cellmap.size();
+    s.ns = sp.ns;
+    for (int k = 0; k < sp.ns; ++k) {
+      s.mass[k] = sp.mass[k];
+      s.charge[k] = sp.charge[k];
+    }
+    s.has_lj = sp.has_lj;
+    if (sp.has_lj)
+      for (int k = 0; k < sp.ns * sp.ns; ++k) {
+        s.lj_eps[k] = sp.eps[k];
+        s.lj_sig2[k] = sp.sig2[k];
+        s.lj_rc2[k] = sp.rc2[k];
+      }
+    s.lj_shift = sp.shift;
+    s.has_coul = sp.has_coul;
+    if (sp.has_coul) {
+      CoulParams& c = s.coul;
+      c.alpha = sp.alpha;
+      c.rc = sp.rcut;
+      c.c = 2 * sp.alpha / std::sqrt(M_PI);
+      c.erc_rc = sp.erc / sp.rcut;
+      c.fshift = sp.erc / (sp.rcut * sp.rcut) + c.c * std::exp(-(sp.alpha * sp.rcut) * (sp.alpha * sp.rcut)) / sp.rcut;
+      c.self = sp.erc / (2 * sp.rcut) + sp.alpha / std::sqrt(M_PI);
+      c.k = sp.kc;
+      c.shifted = 1;
+    }
+    s.rc_list2 = sp.rc_list * sp.rc_list;
+    s.half_skin2 = sp.skin > 0 ? (0.5 * sp.skin) * (0.5 * sp.skin) : -1.0;
+    s.dt = sp.dt;
+    // bricks and brick-major cell numbering
+    const int nbx[3] = {(s.nc[0] + sp.bdim[0] - 1) / sp.bdim[0], (s.nc[1] + sp.bdim[1] - 1) / sp.bdim[1],
+                        (s.nc[2] + sp.bdim[2] - 1) / sp.bdim[2]};
+    cellmap.resize(cellmap.size() + s.ncells, -1);
+    int gid = cell_off;
+    for (int BX = 0; BX < nbx[0]; ++BX)
+      for (int BY = 0; BY < nbx[1]; ++BY)
+        for (int BZ = 0; BZ < nbx[2]; ++BZ) {
+          BBrick b;
+          b.sys = si;
+          b.cell0 = gid;
+          int lo[3] = {BX * sp.bdim[0], BY * sp.bdim[1], BZ * sp.bdim[2]};
+          for (int d = 0; d < 3; ++d) {
+            b.hd[d] = std::min(sp.bdim[d], s.nc[d] - lo[d]);
+            b.rd[d] = b.hd[d] + 2;
+            b.org[d] = (lo[d] + 0.5 * b.hd[d]) * (sp.box[d] / s.nc[d]);
+          }
+          b.nhome = b.hd[0] * b.hd[1] * b.hd[2];
+          ARG_CHECK(b.rd[0] * b.rd[1] * b.rd[2] <= kMaxRegionCells, "brick region too large");
+          for (int lx = 0; lx < b.hd[0]; ++lx)
+            for (int ly = 0; ly < b.hd[1]; ++ly)
+              for (int lz = 0; lz < b.hd[2]; ++lz) {
+                const int cx = lo[0] + lx, cy = lo[1] + ly, cz = lo[2] + lz;
+                cellmap[s.cellmap_off + (cx * s.nc[1] + cy) * s.nc[2] + cz] = gid++;
+                cellsys.push_back(si);
+              }
+          b.reg_off = (int)hr.size();
+          hb.push_back(b);
+          corner.push_back({lo[0], lo[1], lo[2]});
+        }
+    // regions (need the complete cell map of the system)
+    for (int k = s.brick_off; k < (int)hb.size(); ++k) {
+      BBrick& b = hb[k];
+      b.reg_off = (int)hr.size();
+      const int* lo = corner[k].data();
+      for (int rx = 0; rx < b.rd[0]; ++rx)
+        for (int ry = 0; ry < b.rd[1]; ++ry)
+          for (int rz = 0; rz < b.rd[2]; ++rz) {
+            int raw[3] = {lo[0] + rx - 1, lo[1] + ry - 1, lo[2] + rz - 1}, w[3], sh[3];
+            for (int d = 0; d < 3; ++d) {
+              sh[d] = raw[d] < 0 ? -1 : (raw[d] >= s.nc[d] ? 1 : 0);
+              w[d] = raw[d] - sh[d] * s.nc[d];
+            }
+            BRegion r;
+            r.gid = cellmap[s.cellmap_off + (w[0] * s.nc[1] + w[1]) * s.nc[2] + w[2]];
+            r.shift = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
+            hr.push_back(r);
+          }
+    }
+    s.nbricks = (int)hb.size() - s.brick_off;
+    m.brick_off[si + 1] = (int)hb.size();
+    cell_off = gid;
+  }
+  m.ncells = cell_off;
+  m.nbricks = (int)hb.size();
+  HIP_TRY(hipMalloc(&m.d_sys, S * sizeof(BSys)));
+  HIP_TRY(hipMemcpy(m.d_sys, hs.data(), S * sizeof(BSys), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&m.d_bricks, hb.size() * sizeof(BBrick)));
+  HIP_TRY(hipMemcpy(m.d_bricks, hb.data(), hb.size() * sizeof(BBrick), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&m.d_reg, hr.size() * sizeof(BRegion)));
+  HIP_TRY(hipMemcpy(m.d_reg, hr.data(), hr.size() * sizeof(BRegion), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&m.d_cellmap, cellmap.size() * sizeof(int)));
+  HIP_TRY(hipMemcpy(m.d_cellmap, cellmap.data(), cellmap.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&m.d_cellsys, cellsys.size() * sizeof(int)));
+  HIP_TRY(hipMemcpy(m.d_cellsys, cellsys.data(), cellsys.size() * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> sysid(m.N);
+  for (int si = 0; si < S; ++si)
+    for (int a = m.atom_off[si]; a < m.atom_off[si + 1]; ++a) sysid[a] = si;
+  HIP_TRY(hipMalloc(&m.d_sysid, m.N * sizeof(int)));
+  HIP_TRY(hipMemcpy(m.d_sysid, sysid.data(), m.N * sizeof(int), hipMemcpyHostToDevice));
+  return ACCU_OK;
+}
+
+}  // namespace accu
+
+using namespace accu;
+
+extern "C" {
+
+int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* pos_host, const double* vel_host,
+                    const int* sp_host, const double* box_host, int precision) {
+  ARG_CHECK(out && natoms && pos_host && vel_host && sp_host && box_host, "null argument");
+  ARG_CHECK(nsys >= 1, "need at least one system");
+  ARG_CHECK(precision == ACCU_FP64 || precision == ACCU_FP32_PAIR, "unknown precision %d", precision);
+  BMD* m = new BMD();
+  m->S = nsys;
+  m->precision = precision;
+  m->spec.resize(nsys);
+  long long N = 0;
+  for (int s = 0; s < nsys; ++s) {
+    ARG_CHECK(natoms[s] >= 2, "system %d has %d atoms", s, natoms[s]);
+    m->spec[s].natoms = natoms[s];
+    for (int d = 0; d < 3; ++d) m->spec[s].box[d] = box_host[3 * s + d];
+    N += natoms[s];
+  }
+  ARG_CHECK(N < (1LL << 30), "too many atoms");
+  m->N = (int)N;
+  const size_t d4 = (size_t)N * sizeof(double4), i4 = (size_t)N * sizeof(int);
+  void** d4s[] = {(void**)&m->pos, (void**)&m->pos_b, (void**)&m->vel, (void**)&m->F, (void**)&m->ref,
+                  (void**)&m->pos2, (void**)&m->vel2, (void**)&m->F2};
+  for (void** p : d4s) HIP_TRY(hipMalloc(p, d4));
+  void** i4s[] = {(void**)&m->perm, (void**)&m->perm2, (void**)&m->cell, (void**)&m->cell2, (void**)&m->tmp,
+                  (void**)&m->sorted, (void**)&m->nn};
+  for (void** p : i4s) HIP_TRY(hipMalloc(p, i4 + 64));
+  HIP_TRY(hipMalloc(&m->par, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMemset(m->par, 0, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMalloc(&m->flags, (nsys + 1) * sizeof(int) + 64));
+  HIP_TRY(hipMemset(m->flags, 0, (nsys + 1) * sizeof(int) + 64));
+  HIP_TRY(hipMalloc(&m->ints, 64));
+  HIP_TRY(hipMemset(m->ints, 0, 64));
+  HIP_TRY(hipMalloc(&m->nreb, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMemset(m->nreb, 0, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMalloc(&m->log_count, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMemset(m->log_count, 0, nsys * sizeof(int) + 64));
+  std::vector<double4> h4(N);
+  std::vector<int> perm(N);
+  for (long long i = 0; i < N; ++i) {
+    h4[i] = make_double4(pos_host[3 * i], pos_host[3 * i + 1], pos_host[3 * i + 2], (double)sp_host[i]);
+    perm[i] = (int)i;
+  }
+  HIP_TRY(hipMemcpy(m->pos, h4.data(), d4, hipMemcpyHostToDevice));
+  for (long long i = 0; i < N; ++i) h4[i] = make_double4(vel_host[3 * i], vel_host[3 * i + 1], vel_host[3 * i + 2], 0);
+  HIP_TRY(hipMemcpy(m->vel, h4.data(), d4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(m->F, 0, d4));
+  HIP_TRY(hipMemcpy(m->perm, perm.data(), i4, hipMemcpyHostToDevice));
+  HIP_TRY(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+  // species indices are checked when the species table arrives
+  m->spec[0].ns = 0;
+  *out = reinterpret_cast<accu_bmd*>(m);
+  // keep a host copy of the species for validation
+  return ACCU_OK;
+}
+
+int accu_bmd_destroy(accu_bmd* h) {
+  if (h) {
+    BMD* m = reinterpret_cast<BMD*>(h);
+    m->release();
+    delete m;
+  }
+  return ACCU_OK;
+}
+
+int accu_bmd_set_system(accu_bmd* h, int sys, int ns, const double* mass, const double* charge, double dt,
+                        double rc_list, double skin, const int* brick_dims) {
+  ARG_CHECK(h && mass && brick_dims, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(sys >= 0 && sys < m.S, "bad system %d", sys);
+  ARG_CHECK(ns >= 1 && ns <= kMaxSpecies, "number of species %d outside [1, %d]", ns, kMaxSpecies);
+  ARG_CHECK(rc_list > 0 && skin >= 0 && dt != 0, "bad list radius, skin or dt");
+  SysSpec& s = m.spec[sys];
+  s.ns = ns;
+  for (int k = 0; k < ns; ++k) {
+    s.mass[k] = mass[k];
+    s.charge[k] = charge ? charge[k] : 0.0;
+  }
+  s.dt = dt;
+  s.rc_list = rc_list;
+  s.skin = skin;
+  for (int d = 0; d < 3; ++d) s.bdim[d] = brick_dims[d];
+  return ACCU_OK;
+}
+
+int accu_bmd_set_lj(accu_bmd* h, int sys, int ns, const double* eps, const double* sig2, const double* rc2,
+                    double shift) {
+  ARG_CHECK(h && eps && sig2 && rc2, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(sys >= 0 && sys < m.S && ns == m.spec[sys].ns, "bad system or species count");
+  SysSpec& s = m.spec[sys];
+  s.has_lj = true;
+  for (int k = 0; k < ns * ns; ++k) {
+    s.eps[k] = eps[k];
+    s.sig2[k] = sig2[k];
+    s.rc2[k] = rc2[k];
+  }
+  s.shift = shift;
+  return ACCU_OK;
+}
+
+int accu_bmd_set_dsf(accu_bmd* h, int sys, double alpha, double rcut, double erc, double k) {
+  ARG_CHECK(h, "null handle");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(sys >= 0 && sys < m.S, "bad system %d", sys);
+  SysSpec& s = m.spec[sys];
+  s.has_coul = true;
+  s.alpha = alpha;
+  s.rcut = rcut;
+  s.erc = erc;
+  s.kc = k;
+  return ACCU_OK;
+}
+
+// Build tables, sort every system into brick order, build the lists, compute the
+// forces at t = 0 and record them. slack sizes the list and LDS capacities.
+int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
+  ARG_CHECK(h, "null handle");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  bool lj0 = m.spec[0].has_lj, co0 = m.spec[0].has_coul;
+  int maxns = 0;
+  for (int s = 0; s < m.S; ++s) {
+    ARG_CHECK(m.spec[s].ns >= 1 && m.spec[s].rc_list > 0, "system %d is not configured", s);
+    ARG_CHECK(m.spec[s].has_lj == lj0 && m.spec[s].has_coul == co0, "all systems must use the same interactions");
+    ARG_CHECK(lj0 || co0, "no interaction set");
+    maxns = std::max(maxns, m.spec[s].ns);
+  }
+  m.any_lj = lj0;
+  m.any_coul = co0;
+  m.one_sp = maxns == 1;
+  int rc = build_tables(m);
+  if (rc) return rc;
+  HIP_TRY(hipMalloc(&m.count, (m.ncells + 1) * sizeof(int)));
+  HIP_TRY(hipMalloc(&m.start, (m.ncells + 1) * sizeof(int)));
+  HIP_TRY(hipMalloc(&m.cursor, (m.ncells + 1) * sizeof(int)));
+  HIP_TRY(hipMalloc(&m.partials, (size_t)m.nbricks * 5 * sizeof(double)));
+  m.log_cap = std::max(log_cap, 4);
+  HIP_TRY(hipMalloc(&m.log, (size_t)m.S * m.log_cap * kRecW * sizeof(double)));
+  m.stride = (m.N + 63) / 64 * 64;
+  // first binning of every system: measure the region sizes to size the LDS
+  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
+  const int T = 256;
+  hipLaunchKernelGGL(k_b_zero, dim3((m.ncells + T - 1) / T), dim3(T), 0, m.st, m.d_cellsys, m.count, m.ncells, m.flags,
+                     m.S);
+  hipLaunchKernelGGL(k_b_assign, dim3((m.N + T - 1) / T), dim3(T), 0, m.st, m.pos, m.pos_b, m.par, m.d_sysid, m.d_sys,
+                     m.d_cellmap, m.cell, m.count, m.N, m.flags, m.S);
+  hipLaunchKernelGGL(k_b_scan, dim3(1), dim3(1024), 0, m.st, m.count, m.start, m.cursor, m.ncells, m.flags, m.S);
+  LAUNCH_CHECK();
+  std::vector<int> hstart(m.ncells + 1);
+  HIP_TRY(hipMemcpyAsync(hstart.data(), m.start, (m.ncells + 1) * sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  std::vector<BBrick> hb(m.nbricks);
+  HIP_TRY(hipMemcpy(hb.data(), m.d_bricks, m.nbricks * sizeof(BBrick), hipMemcpyDeviceToHost));
+  int regmax = 0;
+  {
+    std::vector<BRegion> hr;
+    size_t nreg = 0;
+    for (auto& b : hb) nreg = std::max(nreg, (size_t)(b.reg_off + b.rd[0] * b.rd[1] * b.rd[2]));
+    hr.resize(nreg);
+    HIP_TRY(hipMemcpy(hr.data(), m.d_reg, nreg * sizeof(BRegion), hipMemcpyDeviceToHost));
+    for (auto& b : hb) {
+      int tot = 0;
+      for (int r = 0; r < b.rd[0] * b.rd[1] * b.rd[2]; ++r) {
+        const int c = hr[b.reg_off + r].gid;
+        tot += hstart[c + 1] - hstart[c];
+      }
+      regmax = std::max(regmax, tot);
+    }
+  }
+  const int cap = (int)(regmax * (1.0 + slack)) + 64;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  const size_t lds_max = prop.sharedMemPerBlock;
+  m.cap_build = cap;
+  m.cap_fused = cap;
+  ARG_CHECK(m.lds_build() + 4096 <= lds_max && cap < 65536,
+            "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
+  (void)hipFuncSetAttribute((const void*)k_b_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
+  // neighbour capacity: measure with a small capacity, then size
+  m.max_nbrs = 1;
+  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * sizeof(unsigned short)));
+  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
+  m.launch_rebuild();
+  int ints[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(ints, m.ints, sizeof ints, hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  ARG_CHECK(ints[1] == 0, "region overflow at the first build (%d atoms)", ints[1]);
+  const int nbrs = (int)(ints[0] * (1.0 + slack)) + 16;
+  HIP_TRY(hipFree(m.nbr));
+  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * nbrs * sizeof(unsigned short)));
+  m.max_nbrs = nbrs;
+  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
+  m.launch_rebuild();
+  // the two measuring builds count as one rebuild per system
+  HIP_TRY(hipMemsetAsync(m.nreb, 0, m.S * sizeof(int), m.st));
+  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
+  hipLaunchKernelGGL(k_b_end, dim3(1), dim3(256), 0, m.st, m.flags, m.S, m.nreb);
+  // forces at t = 0, first record
+  auto set_attr = [&](const void* f) {
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_fused());
+  };
+  set_attr((const void*)k_b_fused<double, true, true, true>);
+  set_attr((const void*)k_b_fused<double, true, false, true>);
+  set_attr((const void*)k_b_fused<double, false, true, true>);
+  set_attr((const void*)k_b_fused<double, true, true, false>);
+  set_attr((const void*)k_b_fused<double, true, false, false>);
+  set_attr((const void*)k_b_fused<double, false, true, false>);
+  set_attr((const void*)k_b_fused<float, true, true, true>);
+  set_attr((const void*)k_b_fused<float, true, false, true>);
+  set_attr((const void*)k_b_fused<float, false, true, true>);
+  set_attr((const void*)k_b_fused<float, true, true, false>);
+  set_attr((const void*)k_b_fused<float, true, false, false>);
+  set_attr((const void*)k_b_fused<float, false, true, false>);
+  m.launch_fused(m.nbricks, 0.0, 0.0, 0, 1);
+  LAUNCH_CHECK();
+  HIP_TRY(hipStreamSynchronize(m.st));
+  m.ready = true;
+  return ACCU_OK;
+}
+
+// coarse steps of nticks ticks each (see BMD::enqueue). use_graph captures one block.
+int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const double* wc, const double* wn,
+                 int record_every, int use_graph) {
+  ARG_CHECK(h && active && wc && wn, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(m.ready, "call accu_bmd_init first");
+  ARG_CHECK(coarse >= 1 && nticks >= 1 && nticks <= 16, "bad step counts");
+  for (int t = 0; t < nticks; ++t) ARG_CHECK(active[t] >= 1 && active[t] <= m.S, "bad active count at tick %d", t);
+  ARG_CHECK(active[nticks - 1] == m.S, "the last tick must advance every system");
+  std::vector<int> cnt(m.S);
+  HIP_TRY(hipMemcpyAsync(cnt.data(), m.log_count, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  const int nrec = record_every > 0 ? coarse / record_every + 2 : 0;
+  for (int s = 0; s < m.S; ++s) ARG_CHECK(cnt[s] + nrec <= m.log_cap, "energy log of system %d is full", s);
+  if (!use_graph) {
+    m.enqueue(coarse, nticks, active, wc, wn, record_every);
+    LAUNCH_CHECK();
+    return ACCU_OK;
+  }
+  std::vector<long long> key = {coarse, nticks, record_every};
+  for (int t = 0; t < nticks; ++t) {
+    key.push_back(active[t]);
+    key.push_back((long long)(wc[t] * 1e15));
+    key.push_back((long long)(wn[t] * 1e15));
+  }
+  if (!(m.gexec && key == m.graph_key)) {
+    if (m.gexec) HIP_TRY(hipGraphExecDestroy(m.gexec));
+    if (m.graph) HIP_TRY(hipGraphDestroy(m.graph));
+    m.gexec = nullptr;
+    m.graph = nullptr;
+    const bool tm = m.timing;
+    m.timing = false;
+    HIP_TRY(hipStreamBeginCapture(m.st, hipStreamCaptureModeThreadLocal));
+    m.enqueue(coarse, nticks, active, wc, wn, record_every);
+    hipError_t e = hipStreamEndCapture(m.st, &m.graph);
+    m.timing = tm;
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return ACCU_ERUNTIME;
+    }
+    HIP_TRY(hipGraphInstantiate(&m.gexec, m.graph, nullptr, nullptr, 0));
+    m.graph_key = key;
+  }
+  HIP_TRY(hipGraphLaunch(m.gexec, m.st));
+  return ACCU_OK;
+}
+
+int accu_bmd_sync(accu_bmd* h) {
+  ARG_CHECK(h, "null handle");
+  HIP_TRY(hipStreamSynchronize(reinterpret_cast<BMD*>(h)->st));
+  return ACCU_OK;
+}
+
+// State in user order (host [N][3], any may be NULL), per-system logs
+// log_host[S][log_max][8] and stats[S][4] = rebuilds, records, 0, 0; info[4] =
+// max neighbours seen, capacity, region capacity, overflowing region (0 = none).
+int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
+                 long long* stats, long long* info) {
+  ARG_CHECK(h, "null handle");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  const int N = m.N;
+  double* tmp = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&tmp, (size_t)N * 3 * sizeof(double), m.st));
+  double4* srcs[3] = {m.pos, m.vel, m.F};
+  double4* alts[3] = {m.pos_b, nullptr, nullptr};
+  double* dsts[3] = {pos_host, vel_host, F_host};
+  for (int k = 0; k < 3; ++k) {
+    if (!dsts[k]) continue;
+    hipLaunchKernelGGL(k_b_scatter_user, dim3((N + 255) / 256), dim3(256), 0, m.st, N, m.perm, srcs[k], alts[k], m.par,
+                       m.d_sysid, tmp);
+    HIP_TRY(hipMemcpyAsync(dsts[k], tmp, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, m.st));
+  }
+  std::vector<int> nreb(m.S), lc(m.S);
+  int ints[2];
+  HIP_TRY(hipMemcpyAsync(nreb.data(), m.nreb, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipMemcpyAsync(lc.data(), m.log_count, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipMemcpyAsync(ints, m.ints, sizeof ints, hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  HIP_TRY(hipFreeAsync(tmp, m.st));
+  if (log_host) {
+    std::vector<double> lg((size_t)m.S * m.log_cap * kRecW);
+    HIP_TRY(hipMemcpy(lg.data(), m.log, lg.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < m.S; ++s)
+      for (int r = 0; r < std::min(log_max, std::min(lc[s], m.log_cap)); ++r)
+        memcpy(log_host + ((size_t)s * log_max + r) * kRecW, lg.data() + ((size_t)s * m.log_cap + r) * kRecW,
+               kRecW * sizeof(double));
+  }
+  if (stats)
+    for (int s = 0; s < m.S; ++s) {
+      stats[4 * s] = nreb[s];
+      stats[4 * s + 1] = std::min(lc[s], m.log_cap);
+      stats[4 * s + 2] = 0;
+      stats[4 * s + 3] = 0;
+    }
+  if (info) {
+    info[0] = ints[0];
+    info[1] = m.max_nbrs;
+    info[2] = m.cap_fused;
+    info[3] = ints[1];
+  }
+  if (ints[1] > 0) {
+    set_error("brick region overflow: %d atoms > LDS capacity %d (raise the slack or use smaller bricks)", ints[1],
+              m.cap_fused);
+    return ACCU_EOVERFLOW;
+  }
+  if (ints[0] > m.max_nbrs) {
+    set_error("neighbour list overflow: %d > capacity %d (raise the slack)", ints[0], m.max_nbrs);
+    return ACCU_EOVERFLOW;
+  }
+  return ACCU_OK;
+}
+
+int accu_bmd_set_timing(accu_bmd* h, int enable) {
+  ARG_CHECK(h, "null handle");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  m.timing = enable != 0;
+  m.ev_used = 0;
+  return ACCU_OK;
+}
+
+int accu_bmd_timing(accu_bmd* h, double* total_ms, long long* launches) {
+  ARG_CHECK(h && total_ms && launches, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  HIP_TRY(hipStreamSynchronize(m.st));
+  double tot = 0.0;
+  for (size_t k = 0; k + 1 < m.ev_used; k += 2) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m.ev[k], m.ev[k + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (long long)(m.ev_used / 2);
+  m.ev_used = 0;
+  return ACCU_OK;
+}
+
+}  // extern "C"

@@ -117,6 +117,37 @@ int accu_md_stream(accu_md* h, void** stream_out);
 int accu_md_set_timing(accu_md* h, int enable);
 int accu_md_timing(accu_md* h, double* total_ms_host, long long* launches_host);
 
+/* ---- brick-tiled multi-system MD engine: the same step as accu_md_* (Simulation.run,
+ * simulation.py:95-112, with VelocityVerlet / Yoshida4, integrators.py:19-47, over LJ
+ * forces.py:119-142 and/or DSF forces.py:408-438, rebuild rule forces.py:86-98) for
+ * one or many periodic systems at once, each with its own box, interactions and dt -
+ * e.g. the dt, dt/2, dt/4 re-runs of timestep_convergence (accuracy.py:191-221).
+ * Systems are given in one concatenated arena (user order); each needs
+ * accu_bmd_set_system and the interaction calls before accu_bmd_init. */
+typedef struct accu_bmd accu_bmd;
+int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms_host, const double* pos_host, const double* vel_host,
+                    const int* sp_host, const double* box_host, int precision);
+int accu_bmd_destroy(accu_bmd* h);
+/* species masses/charges [ns], time step, list radius (incl. skin), skin, brick dims [3] */
+int accu_bmd_set_system(accu_bmd* h, int sys, int ns, const double* mass_host, const double* charge_host, double dt,
+                        double rc_list, double skin, const int* brick_dims_host);
+int accu_bmd_set_lj(accu_bmd* h, int sys, int ns, const double* eps_host, const double* sig2_host,
+                    const double* rc2_host, double shift);
+int accu_bmd_set_dsf(accu_bmd* h, int sys, double alpha, double rcut, double erc, double k);
+int accu_bmd_init(accu_bmd* h, double slack, int log_cap);
+/* coarse steps of nticks ticks; at tick t the first active[t] systems advance one
+ * (sub)step with closing-kick weight wc[t] and next-drift weight wn[t] (times each
+ * system's dt); active[nticks-1] must be nsys; a record every record_every coarse steps */
+int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active_host, const double* wc_host,
+                 const double* wn_host, int record_every, int use_graph);
+int accu_bmd_sync(accu_bmd* h);
+/* state in user order; log_host [nsys][log_max][8]; stats [nsys][4] (rebuilds, records);
+ * info [4] = max neighbours, capacity, region capacity, region overflow */
+int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
+                 long long* stats_host, long long* info_host);
+int accu_bmd_set_timing(accu_bmd* h, int enable);
+int accu_bmd_timing(accu_bmd* h, double* total_ms_host, long long* launches_host);
+
 /* ---- learned DSF -> Ewald pair correction (no counterpart in the reference; it
  * corrects DampedShiftedCoulomb, forces.py:391-452, toward Ewald, forces.py:333-388).
  * A 4 -> 64 -> 64 -> 1 pair MLP; parameters are 4545 float32 values laid out as

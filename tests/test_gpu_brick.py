@@ -1,0 +1,123 @@
+"""GPU parity of the brick-tiled multi-system engine: against the reference's golden
+trajectory and the oracle, batched vs single runs (bitwise), graph replays, mixed
+precision, and the batched Richardson replicas against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import accusim_np as A
+from oracle import builders as B
+
+pytestmark = pytest.mark.gpu
+
+dp = pytest.importorskip("devinproj_amd")
+from devinproj_amd import batch  # noqa: E402
+
+DT = 10e-15
+
+
+def _lj():
+    return [dp.LennardJones(B.argon_params(), cutoff=2.5, skin=0.3)]
+
+
+def test_brick_vv_matches_reference_trajectory(golden):
+    g = golden("argon864_vv20.npz")
+    s = B.argon(6, seed=64)
+    eng = batch.BrickEngine([s], [_lj()], dp.VelocityVerlet(), DT)
+    eng.run(20)
+    eng.fetch()
+    lg = eng.logs[0]
+    assert len(lg) == 21
+    assert np.allclose(lg[:, 0], g["pe"], rtol=1e-10, atol=0)
+    assert np.allclose(lg[:, 1], g["ke"], rtol=1e-10, atol=0)
+    assert np.max(np.abs(s.pos - g["pos"])) <= 1e-10 * float(g["box"][0])
+    assert np.max(np.abs(eng.forces_out[0] - g["F"])) <= 1e-9 * np.max(np.abs(g["F"]))
+    assert int(eng.rebuilds[0]) == int(g["rebuilds"])
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-6)])
+def test_brick_yoshida_matches_oracle(precision, tol):
+    s = B.argon(5, seed=8)
+    so = s.copy()
+    eng = batch.BrickEngine([s], [_lj()], dp.Yoshida4(), DT, precision=precision)
+    eng.run(10)
+    eng.fetch()
+    sim = A.Simulation(so, [A.LennardJones(B.argon_params())], integrator=A.Yoshida4(), dt=DT)
+    tr = sim.run(steps=10, positions=False)
+    assert np.allclose(eng.logs[0][:, 0], tr.potential, rtol=tol, atol=0)
+    assert np.allclose(eng.logs[0][:, 1], tr.kinetic, rtol=tol, atol=0)
+
+
+def test_brick_lj_plus_dsf_matches_oracle():
+    s = B.rocksalt(16, 16, 16, seed=5)
+    rng = np.random.default_rng(1)
+    s.pos = (s.pos + rng.normal(0, 0.2e-10, s.pos.shape)) % s.box
+    so = s.copy()
+    mk = lambda M: [M.LennardJones(B.NACL_LJ, cutoff=2.5), M.DampedShiftedCoulomb(rcut=12e-10, alpha=0.2e10)]
+    eng = batch.BrickEngine([s], [mk(dp)], dp.VelocityVerlet(), 1e-15)
+    eng.run(4)
+    eng.fetch()
+    sim = A.Simulation(so, mk(A), integrator=A.VelocityVerlet(), dt=1e-15)
+    tr = sim.run(steps=4, positions=False)
+    assert np.allclose(eng.logs[0][:, 0], tr.potential, rtol=1e-9, atol=0)
+    assert np.allclose(eng.logs[0][:, 1], tr.kinetic, rtol=1e-9, atol=0)
+    assert np.max(np.abs(s.pos - so.pos)) <= 1e-10 * so.box[0]
+
+
+def test_batched_systems_equal_single_runs_bitwise():
+    systems = [B.lj_reduced(n, rho, seed=k) for k, (n, rho) in enumerate([(4000, 0.8), (6000, 0.75), (5000, 0.85)])]
+    mk = lambda: [dp.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3)]
+    dts = [0.005, 0.0025, 0.004]
+    bricks = [(2, 2, 2)] * 3
+    batched = [s.copy() for s in systems]
+    eb = batch.BrickEngine(batched, [mk() for _ in systems], dp.VelocityVerlet(), dts, bricks=bricks)
+    eb.run(30, record_every=5)
+    eb.fetch()
+    for k, s in enumerate(systems):
+        single = s.copy()
+        es = batch.BrickEngine([single], [mk()], dp.VelocityVerlet(), dts[k], bricks=[bricks[k]])
+        es.run(30, record_every=5)
+        es.fetch()
+        assert np.array_equal(single.pos, batched[k].pos)
+        assert np.array_equal(es.logs[0], eb.logs[k])
+
+
+def test_brick_graph_replays_are_bitwise_identical():
+    s1, s2 = B.argon(8, seed=3), B.argon(8, seed=3)
+    e1 = batch.BrickEngine([s1], [_lj()], dp.VelocityVerlet(), DT)
+    e2 = batch.BrickEngine([s2], [_lj()], dp.VelocityVerlet(), DT)
+    for _ in range(3):
+        e1.run(7)
+        e2.run(7, use_graph=True)
+    e1.fetch()
+    e2.fetch()
+    assert np.array_equal(e1.logs[0], e2.logs[0])
+    assert np.array_equal(s1.pos, s2.pos)
+
+
+def test_brick_mixed_precision_force_error():
+    s64 = B.argon(16, seed=2, jitter=0.01)
+    s32 = s64.copy()
+    e64 = batch.BrickEngine([s64], [_lj()], dp.VelocityVerlet(), DT)
+    e32 = batch.BrickEngine([s32], [_lj()], dp.VelocityVerlet(), DT, precision="mixed")
+    e64.fetch()
+    e32.fetch()
+    err = A.rms_rel_force_error(e32.forces_out[0], e64.forces_out[0])
+    assert err < 1e-5, err
+    e64.run(50)
+    e32.run(50)
+    e64.fetch()
+    e32.fetch()
+    assert abs(e32.energy_drift() - e64.energy_drift()) < 2e-6
+
+
+def test_batched_richardson_matches_oracle():
+    systems = [B.lj_reduced(n, 0.8, seed=k) for k, n in enumerate([500, 864])]
+    make = lambda s: [dp.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3)]
+    checks, eng, _ = batch.richardson([s.copy() for s in systems], make, 0.004, 20, precision="fp64",
+                                      use_graph=True, block=10)
+    for s, chk in zip(systems, checks):
+        sim = A.Simulation(s.copy(), [A.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3)],
+                           integrator=A.VelocityVerlet(), dt=0.004)
+        status, est, p = A.timestep_convergence(sim, 20 * 0.004, start=sim)
+        assert chk.status == status
+        assert chk.value == pytest.approx(est, rel=1e-3)
