@@ -9,6 +9,7 @@ from .engine import MDEngine
 from .forces import (DampedShiftedCoulomb, Ewald, Force, Gravity, LennardJones, NeighbourList, VerletList,
                      cell_list_pairs, kvector_columns)
 from .integrators import Integrator, VelocityVerlet, Yoshida4
+from . import workloads
 from .learned import LearnedDSF, PairCorrection, train_correction
 from .simulation import Simulation, Trajectory
 from .system import Species, System

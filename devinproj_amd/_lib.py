@@ -174,6 +174,7 @@ SIGNATURES.update({
     "accu_bmd_run": [P, I, I, P, P, P, I, I],
     "accu_bmd_sync": [P],
     "accu_bmd_get": [P, P, P, P, P, I, P, P],
+    "accu_bmd_set_threads": [P, I],
     "accu_bmd_set_timing": [P, I],
     "accu_bmd_timing": [P, P, P],
 })

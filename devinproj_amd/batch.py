@@ -40,7 +40,7 @@ def default_bricks(s, rc_list, cap_atoms=1500):
 
 class BrickEngine:
     def __init__(self, systems, forces, integrator=None, dts=None, precision="fp64", bricks=None, slack=0.4,
-                 log_cap=20_000):
+                 log_cap=20_000, threads=256):
         self.systems = list(systems)
         S = len(self.systems)
         if len(forces) != S:
@@ -103,6 +103,8 @@ class BrickEngine:
                 erc = float(_erfc(np.array(f.alpha * f.rcut)))
                 L.call("accu_bmd_set_dsf", h, k, float(f.alpha), float(f.rcut), erc, float(f.k))
         self.log_cap = log_cap
+        self.threads = threads
+        L.call("accu_bmd_set_threads", h, int(threads))
         L.call("accu_bmd_init", h, float(slack), int(log_cap))
         self.steps = 0
 
@@ -129,10 +131,11 @@ class BrickEngine:
         F = np.empty((N, 3)) if state else None
         log = np.zeros((S, self.log_cap, LOG_W))
         stats = np.zeros((S, 4), np.int64)
-        info = np.zeros(4, np.int64)
+        info = np.zeros(5, np.int64)
         L.call("accu_bmd_get", self._h, L.ptr(pos), L.ptr(vel), L.ptr(F), L.ptr(log), self.log_cap, L.ptr(stats),
                L.ptr(info))
-        self.info = {"max_neighbours": int(info[0]), "capacity": int(info[1]), "region_capacity": int(info[2])}
+        self.info = {"max_neighbours": int(info[0]), "capacity": int(info[1]), "region_capacity": int(info[2]),
+                     "list_entries": int(info[4])}
         self.rebuilds = stats[:, 0].copy()
         self.logs = [log[k, :int(stats[k, 1])].copy() for k in range(S)]
         if state:

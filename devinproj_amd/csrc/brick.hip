@@ -29,7 +29,7 @@
 
 namespace accu {
 
-constexpr int kBT = 256;
+constexpr int kBT = 512;  // launch bound; the launch size is the runtime `threads`
 constexpr int kMaxRegionCells = 256;
 constexpr int kRecW = 8;  // PE, KE, Px, Py, Pz, rebuilds, 0, 0
 
@@ -255,7 +255,10 @@ __global__ void k_b_copyback(int natoms, const int* __restrict__ sysid, double4*
 }
 
 // Verlet list of one brick: every home atom against the 27 region cells around its
-// own cell, float64 distance test |d|^2 < rc_list^2, 16-bit local indices.
+// own cell, 16-bit local indices. The engine's list only has to contain every pair
+// within rc_list (a superset is harmless: forces apply their own cutoffs), so the
+// test runs in float32 on brick-relative coordinates with a 1e-5 relative margin on
+// rc_list^2, and region cells whose box is beyond rc_list from the atom are skipped.
 __global__ void __launch_bounds__(kBT) k_b_build(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
                                                  const BRegion* __restrict__ reg, const int* __restrict__ start,
                                                  const double4* __restrict__ posA, const double4* __restrict__ posB,
@@ -268,30 +271,38 @@ __global__ void __launch_bounds__(kBT) k_b_build(const BSys* __restrict__ sys, c
   if (!flags[b.sys]) return;
   const BSys& s = sys[b.sys];
   __shared__ int rs[kMaxRegionCells + 1];
-  extern __shared__ __align__(16) double4 lds64[];
+  extern __shared__ __align__(16) float4 lds32[];
   const int total = region_starts(b, reg, start, rs);
   if (total > cap) {
     if (threadIdx.x == 0) atomicMax(overflow, total);
     return;
   }
-  stage_region<double>(b, s, reg, start, rs, total, cur_buf(posA, posB, par, b.sys), lds64);
+  stage_region<float>(b, s, reg, start, rs, total, cur_buf(posA, posB, par, b.sys), lds32);
   __syncthreads();
+  const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
+  const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
+                       -(1.0f + 0.5f * b.hd[2]) * h[2]};  // low corner of region cell 0
+  const float rc2 = (float)(s.rc_list2 * (1.0 + 1e-5));
   const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
   int mymax = 0;
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
     const int local = cell[a] - b.cell0;
     const int rh = home_region(b, local);
     const int self = rs[rh] + (a - start[b.cell0 + local]);
-    const double4 xi = lds64[self];
+    const float4 xi = lds32[self];
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
     int cnt = 0;
     for (int o = 0; o < 27; ++o) {
-      const int r = ((lx + o / 9) * b.rd[1] + (ly + (o / 3) % 3)) * b.rd[2] + (lz + o % 3);
+      const int rx = lx + o / 9, ry = ly + (o / 3) % 3, rz = lz + o % 3;
+      const float ex = fmaxf(fmaxf(c0[0] + rx * h[0] - xi.x, xi.x - (c0[0] + (rx + 1) * h[0])), 0.f);
+      const float ey = fmaxf(fmaxf(c0[1] + ry * h[1] - xi.y, xi.y - (c0[1] + (ry + 1) * h[1])), 0.f);
+      const float ez = fmaxf(fmaxf(c0[2] + rz * h[2] - xi.z, xi.z - (c0[2] + (rz + 1) * h[2])), 0.f);
+      if (ex * ex + ey * ey + ez * ez > rc2) continue;
+      const int r = (rx * b.rd[1] + ry) * b.rd[2] + rz;
       for (int q = rs[r]; q < rs[r + 1]; ++q) {
-        if (q == self) continue;
-        const double4 xj = lds64[q];
-        const double dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        if (r2_np(dx, dy, dz) < s.rc_list2) {
+        const float4 xj = lds32[q];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        if (dx * dx + dy * dy + dz * dz < rc2 && q != self) {
           if (cnt < max_nbrs) nbr[(size_t)cnt * stride + a] = (unsigned short)q;
           ++cnt;
         }
@@ -371,7 +382,7 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   const BSys& s = sys[b.sys];
   __shared__ int rs[kMaxRegionCells + 1];
   __shared__ P ljt[3 * 64 + 1];
-  __shared__ double red[5 * 4];
+  __shared__ double red[5 * 8];
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T4* lds = reinterpret_cast<T4*>(lds_raw);
   const double4* __restrict__ pos = par[b.sys] ? posB : posA;
@@ -534,7 +545,7 @@ __global__ void k_b_record(const BSys* __restrict__ sys, const double* __restric
                            const int* __restrict__ nreb) {
   const int si = blockIdx.x;
   const BSys& s = sys[si];
-  __shared__ double red[5 * 4];
+  __shared__ double red[5 * 8];
   double acc[5] = {0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
     for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
@@ -618,6 +629,7 @@ struct BMD {
   int log_cap = 0;
   double* partials = nullptr;
   int cap_fused = 0, cap_build = 0;  // region capacity (atoms) for the LDS
+  int threads = 256;                 // workgroup size of the per-brick kernels (64..512)
   bool ready = false;
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -639,7 +651,7 @@ struct BMD {
   }
 
   size_t lds_fused() const { return (size_t)cap_fused * (precision == ACCU_FP64 ? sizeof(double4) : sizeof(float4)); }
-  size_t lds_build() const { return (size_t)cap_build * sizeof(double4); }
+  size_t lds_build() const { return (size_t)cap_build * sizeof(float4); }
 
   void launch_rebuild() {
     const int T = 256;
@@ -654,7 +666,7 @@ struct BMD {
                        perm, cell, pos2, vel2, F2, perm2, cell2, flags, S);
     hipLaunchKernelGGL(k_b_copyback, dim3((N + T - 1) / T), dim3(T), 0, st, N, d_sysid, pos, pos_b, par, ref, vel, F,
                        perm, cell, pos2, vel2, F2, perm2, cell2, flags, S);
-    hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(kBT), lds_build(), st, d_sys, d_bricks, d_reg, start, pos, pos_b,
+    hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(threads), lds_build(), st, d_sys, d_bricks, d_reg, start, pos, pos_b,
                        par, cell, nn, nbr, stride, max_nbrs, cap_build, ints, ints + 1, flags, S);
     hipLaunchKernelGGL(k_b_end, dim3(1), dim3(256), 0, st, flags, S, nreb);
   }
@@ -663,15 +675,15 @@ struct BMD {
   void fused_t(int nb, double wc, double wn, int drift_from, int rec, bool lj, bool co) {
     const size_t sh = lds_fused();
     if (lj && co)
-      hipLaunchKernelGGL((k_b_fused<P, true, true, ONE>), dim3(nb), dim3(kBT), sh, st, d_sys, d_bricks, d_reg, start,
+      hipLaunchKernelGGL((k_b_fused<P, true, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
                          pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
                          flags, S, partials, ints + 1);
     else if (lj)
-      hipLaunchKernelGGL((k_b_fused<P, true, false, ONE>), dim3(nb), dim3(kBT), sh, st, d_sys, d_bricks, d_reg, start,
+      hipLaunchKernelGGL((k_b_fused<P, true, false, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
                          pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
                          flags, S, partials, ints + 1);
     else
-      hipLaunchKernelGGL((k_b_fused<P, false, true, ONE>), dim3(nb), dim3(kBT), sh, st, d_sys, d_bricks, d_reg, start,
+      hipLaunchKernelGGL((k_b_fused<P, false, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
                          pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
                          flags, S, partials, ints + 1);
   }
@@ -1187,6 +1199,12 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
     info[1] = m.max_nbrs;
     info[2] = m.cap_fused;
     info[3] = ints[1];
+    // [4] = current list entries (both directions) over all systems
+    std::vector<int> nn(N);
+    HIP_TRY(hipMemcpy(nn.data(), m.nn, (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
+    long long tot = 0;
+    for (int v : nn) tot += v;
+    info[4] = tot;
   }
   if (ints[1] > 0) {
     set_error("brick region overflow: %d atoms > LDS capacity %d (raise the slack or use smaller bricks)", ints[1],
@@ -1196,6 +1214,23 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
   if (ints[0] > m.max_nbrs) {
     set_error("neighbour list overflow: %d > capacity %d (raise the slack)", ints[0], m.max_nbrs);
     return ACCU_EOVERFLOW;
+  }
+  return ACCU_OK;
+}
+
+// Workgroup size of the per-brick kernels (a multiple of 64 in [64, 512]).
+int accu_bmd_set_threads(accu_bmd* h, int threads) {
+  ARG_CHECK(h, "null handle");
+  ARG_CHECK(threads >= 64 && threads <= kBT && threads % 64 == 0, "threads %d not a multiple of 64 in [64, %d]",
+            threads, kBT);
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  m.threads = threads;
+  if (m.gexec) {
+    (void)hipGraphExecDestroy(m.gexec);
+    (void)hipGraphDestroy(m.graph);
+    m.gexec = nullptr;
+    m.graph = nullptr;
+    m.graph_key.clear();
   }
   return ACCU_OK;
 }

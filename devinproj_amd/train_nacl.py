@@ -1,0 +1,119 @@
+"""Train the learned DSF -> Ewald correction on molten NaCl, entirely on the GPU.
+
+    python -m devinproj_amd.train_nacl [--cells 32 32 64] [--snapshots 200] [--epochs 10]
+                                       [--out path.npz]
+
+1. Melt the rocksalt lattice with the brick engine (LJ + DSF, NVE from a hot start,
+   velocities rescaled to the target temperature), then sample snapshots.
+2. Targets: Ewald forces (real-space error 1e-6: alpha rc = 3.459, reciprocal
+   cutoff at the same error) minus DSF forces, both from the GPU Force classes.
+3. Fit the pair MLP with the GPU forward/backward kernels and Adam.
+4. Report, on held-out snapshots, the RMS relative force error against Ewald of DSF
+   alone and of DSF + correction (accuracy.py's sqrt(mean|dF|^2 / mean|F|^2)).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .accuracy import rms_rel_error
+from .batch import BrickEngine
+from .forces import DampedShiftedCoulomb, Ewald, LennardJones
+from .integrators import VelocityVerlet
+from .learned import PairCorrection, train_correction
+from .units import k_B
+from .workloads import NACL_LJ, rocksalt
+
+RC = 12e-10
+ALPHA_DSF = 0.2e10
+
+
+def ewald_1e6(rc=RC):
+    a = 3.459 / rc
+    return Ewald(alpha=a, rcut=rc, kcut=2 * a * np.sqrt(np.log(1e6)))
+
+
+def melt_and_sample(cells, n_snap, every, T=1200.0, T_hot=3000.0, melt_steps=2000, equil_steps=1000, seed=3,
+                    log=None):
+    s = rocksalt(*cells, T=T_hot, seed=seed)
+    forces = lambda: [LennardJones(NACL_LJ, cutoff=2.5, precision="mixed"),
+                      DampedShiftedCoulomb(rcut=RC, alpha=ALPHA_DSF, precision="mixed")]
+    eng = BrickEngine([s], [forces()], VelocityVerlet(), 1e-15, precision="mixed", bricks=[(1, 1, 1)])
+    eng.run(melt_steps, record_every=100)
+    eng.fetch()
+    eng.close()
+    # rescale to the target temperature and equilibrate
+    ke = 0.5 * np.sum(s.masses[:, None] * s.vel ** 2)
+    T_now = 2 * ke / (3 * (s.n - 1) * k_B)
+    s.vel *= np.sqrt(T / T_now)
+    eng = BrickEngine([s], [forces()], VelocityVerlet(), 1e-15, precision="mixed", bricks=[(1, 1, 1)])
+    eng.run(equil_steps, record_every=100)
+    snaps = []
+    for k in range(n_snap):
+        eng.run(every, record_every=10 ** 9)
+        eng.fetch()
+        snaps.append(s.copy())
+    eng.close()
+    if log:
+        log(f"melted from {T_hot} K (T before rescale {T_now:.0f} K), {n_snap} snapshots every {every} steps")
+    return snaps
+
+
+def targets(snaps, log=None):
+    ew = ewald_1e6()
+    dsf = DampedShiftedCoulomb(rcut=RC, alpha=ALPHA_DSF)
+    Fe, Fd = [], []
+    for s in snaps:
+        Fe.append(ew.compute(s)[0])
+        Fd.append(dsf.compute(s)[0])
+    return Fe, Fd
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cells", type=int, nargs=3, default=[32, 32, 64])
+    ap.add_argument("--snapshots", type=int, default=200)
+    ap.add_argument("--heldout", type=int, default=20)
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    L.lib()
+    t0 = time.perf_counter()
+    snaps = melt_and_sample(tuple(a.cells), a.snapshots + a.heldout, a.every, log=log)
+    t1 = time.perf_counter()
+    Fe, Fd = targets(snaps)
+    t2 = time.perf_counter()
+    dF = [e - d for e, d in zip(Fe, Fd)]
+    F0 = float(np.sqrt(np.mean([np.mean(np.sum(x ** 2, 1)) for x in dF[:a.snapshots]])))
+    corr = PairCorrection(RC, 2.8e-10, 3.8e-10, n_scale=8.0, F0=F0)
+    train, test = list(range(a.snapshots)), list(range(a.snapshots, a.snapshots + a.heldout))
+    hist = train_correction(corr, [snaps[k] for k in train], [dF[k] for k in train], epochs=a.epochs, lr=a.lr,
+                            log=log)
+    t3 = time.perf_counter()
+    err_dsf, err_corr = [], []
+    for k in test:
+        Fc, _ = corr.compute(snaps[k])
+        err_dsf.append(rms_rel_error(Fd[k], Fe[k]))
+        err_corr.append(rms_rel_error(Fd[k] + Fc, Fe[k]))
+    res = {"ions": snaps[0].n, "train_snapshots": len(train), "heldout": len(test), "epochs": a.epochs,
+           "F0_newton": F0, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+           "rms_rel_error_dsf_vs_ewald": float(np.mean(err_dsf)),
+           "rms_rel_error_dsf_plus_correction_vs_ewald": float(np.mean(err_corr)),
+           "seconds": {"melt_and_sample": t1 - t0, "ewald_targets": t2 - t1, "training": t3 - t2}}
+    print(json.dumps(res), flush=True)
+    if a.out:
+        np.savez(a.out, params=corr.params, F0=F0, rc=RC, r_on=2.8e-10, r_off=3.8e-10, n_scale=8.0,
+                 meta=json.dumps(res))
+    return res, corr
+
+
+if __name__ == "__main__":
+    main()

@@ -142,9 +142,11 @@ int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active_host, co
                  const double* wn_host, int record_every, int use_graph);
 int accu_bmd_sync(accu_bmd* h);
 /* state in user order; log_host [nsys][log_max][8]; stats [nsys][4] (rebuilds, records);
- * info [4] = max neighbours, capacity, region capacity, region overflow */
+ * info [5] = max neighbours, capacity, region capacity, region overflow, list entries */
 int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
                  long long* stats_host, long long* info_host);
+/* workgroup size of the per-brick kernels (multiple of 64 in [64, 512]; default 256) */
+int accu_bmd_set_threads(accu_bmd* h, int threads);
 int accu_bmd_set_timing(accu_bmd* h, int enable);
 int accu_bmd_timing(accu_bmd* h, double* total_ms_host, long long* launches_host);
 

@@ -48,7 +48,8 @@ def test_brick_yoshida_matches_oracle(precision, tol):
 
 
 def test_brick_lj_plus_dsf_matches_oracle():
-    s = B.rocksalt(16, 16, 16, seed=5)
+    # 13,824 ions, 5 cells per axis at rc + skin = 13.3 A: a brick's region fits the LDS
+    s = B.rocksalt(24, 24, 24, seed=5)
     rng = np.random.default_rng(1)
     s.pos = (s.pos + rng.normal(0, 0.2e-10, s.pos.shape)) % s.box
     so = s.copy()
