@@ -217,7 +217,9 @@ __global__ void __launch_bounds__(256) k_mlp_force(const double* __restrict__ po
     const double xi = pos[(size_t)ps * i], yi = pos[(size_t)ps * i + 1], zi = pos[(size_t)ps * i + 2];
     const float qi = q[i], ni = coord[i];
     const int cnt = nn[t];
-    float fx = 0.f, fy = 0.f, fz = 0.f;
+    // float64 per-lane sums: s * dx of two floats is exact in float64, so pair terms
+    // (i, j) and (j, i) cancel to float64 rounding and the total force stays ~0
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     int qc = 0;
     for (int base = 0; base < cnt || qc > 0; base += 64) {
       if (base < cnt) qc = enqueue_chunk(base, cnt, i, t, pos, ps, nbr, stride, box, g, xi, yi, zi, qb, qc);
@@ -231,9 +233,9 @@ __global__ void __launch_bounds__(256) k_mlp_force(const double* __restrict__ po
         tile_forward(sh, aW2, hk, st);
         const float s = st.s_raw * st.env * __builtin_amdgcn_rcpf(st.r);
         if (hk == 0 && valid) {
-          fx = fmaf(s, e.dx, fx);
-          fy = fmaf(s, e.dy, fy);
-          fz = fmaf(s, e.dz, fz);
+          fx += (double)s * (double)e.dx;
+          fy += (double)s * (double)e.dy;
+          fz += (double)s * (double)e.dz;
         }
         // shift the queue down by m
         const int rest = qc - m;
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(256) k_mlp_force(const double* __restrict__ po
         qc = rest;
       }
     }
-    double sx = wave_sum((double)fx), sy = wave_sum((double)fy), sz = wave_sum((double)fz);
+    double sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
     if (lane == 0) {
       const double f0 = (double)g.F0;
       if (accumulate) {
