@@ -1,6 +1,7 @@
 """Benchmark: MD steps/s of the device-resident fused LJ path on 1,048,576 argon atoms.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-dsf] [--no-fp64] [--no-learned]
+                    [--no-graph] [--copy-probe MIB]
 
 Workload (one step = one velocity-Verlet step of the whole system, state resident in
 HBM, energies recorded every step): 64^3 x 4 = 1,048,576-atom FCC argon at
@@ -130,6 +131,18 @@ def learned_throughput(reps=5):
             "note": "drop-in compute() wall times including host<->device copies"}
 
 
+def copy_probe(mib=1024, reps=3):
+    """Plain device-to-device copies of known size: the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes `mib` MiB)."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copy_mib": mib, "reps": reps}
+
+
 def pairs_within(s, rc):
     import devinproj_amd as dp
     i, _ = dp.cell_list_pairs(s, rc)
@@ -162,6 +175,9 @@ def main():
     ap.add_argument("--no-dsf", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph timing")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run plain copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path", file=sys.stderr)
@@ -173,13 +189,16 @@ def main():
     _lib.lib()
     name = torch.cuda.get_device_name(0)
     log(f"device: {name}")
+    if args.copy_probe:
+        log(f"copy probe: {copy_probe(args.copy_probe)}")
     s = W.argon(64, seed=2, jitter=0.01)
     n = s.n
     npairs = pairs_within(s, 2.5 * W.AR_SIGMA)
     log(f"workload: {n} atoms, box {s.box[0] * 1e10:.2f} A, {npairs} pairs within rc")
-    mixed = brick_lj(s.copy(), "mixed", args.steps, args.warmup, graph_steps=args.steps)
+    gsteps = 0 if args.no_graph else args.steps
+    mixed = brick_lj(s.copy(), "mixed", args.steps, args.warmup, graph_steps=gsteps)
     log(f"mixed: {mixed}")
-    fp64 = None if args.no_fp64 else brick_lj(s.copy(), "fp64", args.steps, args.warmup, graph_steps=args.steps)
+    fp64 = None if args.no_fp64 else brick_lj(s.copy(), "fp64", args.steps, args.warmup, graph_steps=gsteps)
     log(f"fp64: {fp64}")
     nacl = None
     if not args.no_dsf:

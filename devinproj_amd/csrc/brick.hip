@@ -128,13 +128,126 @@ __device__ __forceinline__ int home_region(const BBrick& b, int local) {
   return ((lx + 1) * b.rd[1] + (ly + 1)) * b.rd[2] + (lz + 1);
 }
 
-// ------------------------------------------------------------------ rebuild chain
+// ------------------------------------------------------------------ rebuild (one launch)
+//
+// Cell binning, counting sort and permutation into brick order of every flagged system
+// run as the phases of ONE persistent launch separated by grid barriers (the grid is
+// sized from the occupancy so that every block is resident); the Verlet lists follow
+// in a second launch with one block per brick. Steps without a rebuild therefore cost
+// two launches that exit at once. The rebuild launch also carries the step's
+// bookkeeping: it flips the position-buffer parity of the systems that just drifted
+// and hands the rebuild flags to the list launch.
 
-__global__ void k_b_zero(const int* __restrict__ cellsys, int* __restrict__ count, int ncells,
-                         const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncells && flags[cellsys[c]]) count[c] = 0;
+constexpr int kRB = 1024;  // threads per block of the rebuild launch: one block per CU, so that
+                           // few blocks pay the cache write-back of each barrier
+constexpr int kBarSpin = 1 << 20;  // bounded barrier wait (0.1 s or more): a stuck barrier reports an error
+
+struct RB {
+  const BSys* sys;
+  const BBrick* bricks;
+  const BRegion* reg;
+  const int* cellmap;
+  const int* cellsys;
+  const int* sysid;
+  double4* posA;
+  double4* posB;
+  int* par;
+  double4* ref;
+  double4* vel;
+  double4* F;
+  int* perm;
+  int* cell;
+  double4* pos2;
+  double4* vel2;
+  double4* F2;
+  int* perm2;
+  int* cell2;
+  int* count;   // per cell; all zero between rebuilds
+  int* start;   // [ncells + 1]
+  int* cursor;  // [ncells + 1]
+  int* tmp;
+  int* sorted;
+  int* bsum;    // [grid]
+  int* nn;
+  uint4* nbr;   // groups of 8 u16 local indices: group g of atom a at [g * stride + a]
+  int stride, ngroups, cap;
+  int* ints;    // [0] max neighbours, [1] region overflow, [2] barrier timeout, [3] blocks seen then
+  unsigned* bar;
+  int* flags;   // [S] per system + [S] any, set by the fused step
+  int* bflags;  // the same, as handed to the list launch
+  int* nreb;
+  int S, N, ncells, nbricks;
+  int flip_n;    // systems [0, flip_n) drifted: their parity flips with this launch
+  int bin_only;  // 1: stop after the scan (set-up measures the regions)
+};
+
+// Grid-wide barrier. Returns false when the wait times out: err[0] is then set (every
+// block leaves and the engine reports the error at the next get) and err[1] keeps the
+// largest number of blocks seen at the barrier.
+__device__ bool grid_sync(unsigned* __restrict__ bar, int* __restrict__ err) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int good = 1;
+    // release this block's writes once; the polling loads below are relaxed so that the
+    // waiting blocks do not keep invalidating the caches of the working ones
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int spins = 0;
+      while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++spins > kBarSpin) {
+          good = 0;
+          atomicOr(err, 1);
+          atomicMax(err + 1, (int)__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other blocks' writes in
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Sum of v over the block (every thread gets it).
+__device__ __forceinline__ int block_sum_int(int v, int* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if (lane == 0) wsum[w] = v;
+  __syncthreads();
+  int t = 0;
+  for (int k = 0; k < nw; ++k) t += wsum[k];
+  return t;
+}
+
+// Exclusive prefix of v over the block; total receives the block sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int woff = 0, tot = 0;
+  for (int k = 0; k < nw; ++k) {
+    woff += k < w ? wsum[k] : 0;
+    tot += wsum[k];
+  }
+  total = tot;
+  return woff + x - v;
 }
 
 // Each system's current positions live in posA or posB according to par[system]
@@ -143,190 +256,264 @@ __device__ __forceinline__ const double4* cur_buf(const double4* A, const double
   return par[s] ? B : A;
 }
 
-__global__ void k_b_assign(const double4* __restrict__ posA, const double4* __restrict__ posB,
-                           const int* __restrict__ par, const int* __restrict__ sysid, const BSys* __restrict__ sys,
-                           const int* __restrict__ cellmap, int* __restrict__ cell, int* __restrict__ count, int natoms,
-                           const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= natoms) return;
-  const int si = sysid[a];
-  if (!flags[si]) return;
-  const BSys& s = sys[si];
-  const double4 x = cur_buf(posA, posB, par, si)[a];
-  const double xs[3] = {x.x, x.y, x.z};
-  int c[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double t = xs[d] * s.cinv[d];
-    const int m = s.nc[d] - 1;
-    c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : (int)t);
-  }
-  const int gid = cellmap[s.cellmap_off + (c[0] * s.nc[1] + c[1]) * s.nc[2] + c[2]];
-  cell[a] = gid;
-  atomicAdd(&count[gid], 1);
-}
-
-__global__ void __launch_bounds__(1024) k_b_scan(const int* __restrict__ in, int* __restrict__ out,
-                                                 int* __restrict__ out2, int n, const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  __shared__ long long s[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int a = t * per, b = (a + per < n) ? a + per : n;
-  long long sum = 0;
-  for (int i = a; i < b; ++i) sum += in[i];
-  s[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    long long v = (t >= o) ? s[t - o] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
-  }
-  long long run = s[t] - sum;
-  for (int i = a; i < b; ++i) {
-    const int v = in[i];
-    out[i] = (int)run;
-    out2[i] = (int)run;
-    run += v;
-  }
-  if (t == 1023) {
-    out[n] = (int)s[1023];
-    out2[n] = (int)s[1023];
-  }
-}
-
-__global__ void k_b_scatter(const int* __restrict__ cell, const int* __restrict__ sysid, int natoms,
-                            int* __restrict__ cursor, int* __restrict__ tmp, const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= natoms || !flags[sysid[a]]) return;
-  tmp[atomicAdd(&cursor[cell[a]], 1)] = a;
-}
-
-__global__ void k_b_sort(const int* __restrict__ start, const int* __restrict__ cellsys, int ncells,
-                         const int* __restrict__ tmp, int* __restrict__ sorted, const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (c >= ncells || !flags[cellsys[c]]) return;
-  const int lane = threadIdx.x & 63;
-  const int a = start[c], len = start[c + 1] - a;
-  for (int e = lane; e < len; e += 64) {
-    const int v = tmp[a + e];
-    int r = 0;
-    for (int x = 0; x < len; ++x) r += (tmp[a + x] < v);
-    sorted[a + r] = v;
-  }
-}
-
-__global__ void k_b_permute(int natoms, const int* __restrict__ sysid, const int* __restrict__ sorted,
-                            const double4* __restrict__ posA, const double4* __restrict__ posB,
-                            const int* __restrict__ par, const double4* __restrict__ vel,
-                            const double4* __restrict__ F, const int* __restrict__ perm, const int* __restrict__ cell,
-                            double4* __restrict__ pos2, double4* __restrict__ vel2, double4* __restrict__ F2,
-                            int* __restrict__ perm2, int* __restrict__ cell2, const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= natoms || !flags[sysid[t]]) return;
-  const int i = sorted[t];
-  pos2[t] = cur_buf(posA, posB, par, sysid[t])[i];
-  vel2[t] = vel[i];
-  F2[t] = F[i];
-  perm2[t] = perm[i];
-  cell2[t] = cell[i];
-}
-
-__global__ void k_b_copyback(int natoms, const int* __restrict__ sysid, double4* __restrict__ posA,
-                             double4* __restrict__ posB, const int* __restrict__ par, double4* __restrict__ ref, double4* __restrict__ vel, double4* __restrict__ F,
-                             int* __restrict__ perm, int* __restrict__ cell, const double4* __restrict__ pos2,
-                             const double4* __restrict__ vel2, const double4* __restrict__ F2,
-                             const int* __restrict__ perm2, const int* __restrict__ cell2,
-                             const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= natoms || !flags[sysid[t]]) return;
-  (par[sysid[t]] ? posB : posA)[t] = pos2[t];
-  ref[t] = pos2[t];
-  vel[t] = vel2[t];
-  F[t] = F2[t];
-  perm[t] = perm2[t];
-  cell[t] = cell2[t];
-}
-
-// Verlet list of one brick: every home atom against the 27 region cells around its
-// own cell, 16-bit local indices. The engine's list only has to contain every pair
-// within rc_list (a superset is harmless: forces apply their own cutoffs), so the
-// test runs in float32 on brick-relative coordinates with a 1e-5 relative margin on
-// rc_list^2, and region cells whose box is beyond rc_list from the atom are skipped.
-__global__ void __launch_bounds__(kBT) k_b_build(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
-                                                 const BRegion* __restrict__ reg, const int* __restrict__ start,
-                                                 const double4* __restrict__ posA, const double4* __restrict__ posB,
-                                                 const int* __restrict__ par, const int* __restrict__ cell,
-                                                 int* __restrict__ nn, unsigned short* __restrict__ nbr, int stride,
-                                                 int max_nbrs, int cap, int* __restrict__ maxcount,
-                                                 int* __restrict__ overflow, const int* __restrict__ flags, int S) {
-  if (flags[S] == 0) return;
-  const BBrick b = bricks[blockIdx.x];
-  if (!flags[b.sys]) return;
-  const BSys& s = sys[b.sys];
-  __shared__ int rs[kMaxRegionCells + 1];
-  extern __shared__ __align__(16) float4 lds32[];
-  const int total = region_starts(b, reg, start, rs);
-  if (total > cap) {
-    if (threadIdx.x == 0) atomicMax(overflow, total);
+// Verlet list of one brick: every home atom against the region cells around its own
+// cell, 16-bit local indices in groups of 8, the last group padded with the index of
+// the far sentinel slot (`total`). The list only has to contain every pair within
+// rc_list (a superset is harmless: forces apply their own cutoffs), so the test runs
+// in float32 on brick-relative coordinates with a 1e-5 relative margin on rc_list^2,
+// and region cells whose box is beyond rc_list from the atom are skipped.
+__device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs, int& mymax) {
+  const BSys& s = p.sys[b.sys];
+  const int total = region_starts(b, p.reg, p.start, rs);
+  if (total + 1 > p.cap) {
+    if (threadIdx.x == 0) atomicMax(&p.ints[1], total + 1);
+    __syncthreads();  // rs is rewritten by the next brick of this block
     return;
   }
-  stage_region<float>(b, s, reg, start, rs, total, cur_buf(posA, posB, par, b.sys), lds32);
+  stage_region<float>(b, s, p.reg, p.start, rs, total, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
   __syncthreads();
   const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
                        -(1.0f + 0.5f * b.hd[2]) * h[2]};  // low corner of region cell 0
   const float rc2 = (float)(s.rc_list2 * (1.0 + 1e-5));
-  const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
-  int mymax = 0;
+  const unsigned sentinel = (unsigned)total;
+  const int a0 = p.start[b.cell0], a1 = p.start[b.cell0 + b.nhome];
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-    const int local = cell[a] - b.cell0;
+    const int local = p.cell[a] - b.cell0;
     const int rh = home_region(b, local);
-    const int self = rs[rh] + (a - start[b.cell0 + local]);
+    const int self = rs[rh] + (a - p.start[b.cell0 + local]);
     const float4 xi = lds32[self];
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
+    // 8 entries per 16-byte group: each hit shifts the 128-bit register down by 16 bits
+    // and enters at the top, so the first entry of a group ends in the low half of w0
     int cnt = 0;
-    for (int o = 0; o < 27; ++o) {
-      const int rx = lx + o / 9, ry = ly + (o / 3) % 3, rz = lz + o % 3;
+    unsigned w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    for (int ox = 0; ox < 3; ++ox) {
+      const int rx = lx + ox;
       const float ex = fmaxf(fmaxf(c0[0] + rx * h[0] - xi.x, xi.x - (c0[0] + (rx + 1) * h[0])), 0.f);
-      const float ey = fmaxf(fmaxf(c0[1] + ry * h[1] - xi.y, xi.y - (c0[1] + (ry + 1) * h[1])), 0.f);
-      const float ez = fmaxf(fmaxf(c0[2] + rz * h[2] - xi.z, xi.z - (c0[2] + (rz + 1) * h[2])), 0.f);
-      if (ex * ex + ey * ey + ez * ez > rc2) continue;
-      const int r = (rx * b.rd[1] + ry) * b.rd[2] + rz;
-      for (int q = rs[r]; q < rs[r + 1]; ++q) {
-        const float4 xj = lds32[q];
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        if (dx * dx + dy * dy + dz * dz < rc2 && q != self) {
-          if (cnt < max_nbrs) nbr[(size_t)cnt * stride + a] = (unsigned short)q;
-          ++cnt;
+      for (int oy = 0; oy < 3; ++oy) {
+        const int ry = ly + oy;
+        const float ey = fmaxf(fmaxf(c0[1] + ry * h[1] - xi.y, xi.y - (c0[1] + (ry + 1) * h[1])), 0.f);
+        for (int oz = 0; oz < 3; ++oz) {
+          const int rz = lz + oz;
+          const float ez = fmaxf(fmaxf(c0[2] + rz * h[2] - xi.z, xi.z - (c0[2] + (rz + 1) * h[2])), 0.f);
+          if (ex * ex + ey * ey + ez * ez > rc2) continue;  // the cell's box is out of range
+          const int r = (rx * b.rd[1] + ry) * b.rd[2] + rz;
+          const int qe = rs[r + 1];
+          for (int q = rs[r]; q < qe; ++q) {
+            const float4 xj = lds32[q];
+            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            if (dx * dx + dy * dy + dz * dz < rc2 && q != self) {
+              w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+              w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+              w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+              w3 = __builtin_amdgcn_alignbit((unsigned)q, w3, 16);
+              if ((cnt & 7) == 7 && (cnt >> 3) < p.ngroups)
+                p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+              ++cnt;
+            }
+          }
         }
       }
     }
-    nn[a] = cnt < max_nbrs ? cnt : max_nbrs;
+    if (cnt & 7) {  // pad the last group with the sentinel
+      for (int e = cnt & 7; e < 8; ++e) {
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+        w3 = __builtin_amdgcn_alignbit(sentinel, w3, 16);
+      }
+      if ((cnt >> 3) < p.ngroups) p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+    }
+    const int capn = p.ngroups * 8;
+    p.nn[a] = cnt < capn ? cnt : capn;
     mymax = cnt > mymax ? cnt : mymax;
   }
-  // one atomic per wave
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(maxcount, mymax);
+  __syncthreads();  // the LDS is restaged by the next brick of this block
 }
 
-__global__ void k_b_end(int* __restrict__ flags, int S, int* __restrict__ nreb) {
-  if (flags[S] == 0) return;
-  for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    if (flags[s]) {
-      nreb[s] += 1;
-      flags[s] = 0;
+// Parity of system si as seen inside the rebuild launch (the flip commits at its end).
+__device__ __forceinline__ int rb_par(const RB& p, int si) { return p.par[si] ^ (si < p.flip_n ? 1 : 0); }
+
+__global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
+  const int tid = threadIdx.x, bd = blockDim.x;
+  if (p.flags[p.S] == 0 || p.ints[2] != 0) {
+    if (blockIdx.x == 0) {  // nothing to rebuild: commit the flip, the list launch exits
+      for (int s = tid; s < p.flip_n; s += bd) p.par[s] ^= 1;
+      if (tid == 0) p.bflags[p.S] = 0;
+    }
+    return;
+  }
+  __shared__ int wsum[kRB / 64];
+  const int gtid = blockIdx.x * bd + tid, gsz = gridDim.x * bd;
+  // 1. cell of every atom of a flagged system; cells of the other systems keep their counts
+  for (int a = gtid; a < p.N; a += gsz) {
+    const int si = p.sysid[a];
+    if (!p.flags[si]) continue;
+    const BSys& s = p.sys[si];
+    const double4 x = (rb_par(p, si) ? p.posB : p.posA)[a];
+    const double xs[3] = {x.x, x.y, x.z};
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t = xs[d] * s.cinv[d];
+      const int m = s.nc[d] - 1;
+      c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : (int)t);
+    }
+    const int gid = p.cellmap[s.cellmap_off + (c[0] * s.nc[1] + c[1]) * s.nc[2] + c[2]];
+    p.cell[a] = gid;
+    atomicAdd(&p.count[gid], 1);
+  }
+  for (int c = gtid; c < p.ncells; c += gsz)
+    if (!p.flags[p.cellsys[c]]) p.count[c] = p.start[c + 1] - p.start[c];
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  // 2. exclusive scan of the counts over all cells (chunk per block), counts back to 0
+  const int chunk = (p.ncells + gridDim.x - 1) / gridDim.x;
+  const int c0 = min((int)blockIdx.x * chunk, p.ncells), c1 = min(c0 + chunk, p.ncells);
+  {
+    int v = 0;
+    for (int c = c0 + tid; c < c1; c += bd) v += p.count[c];
+    v = block_sum_int(v, wsum);
+    if (tid == 0) p.bsum[blockIdx.x] = v;
+  }
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  {
+    int off = 0;
+    for (int k = tid; k < (int)blockIdx.x; k += bd) off += p.bsum[k];
+    off = block_sum_int(off, wsum);
+    for (int base = c0; base < c1; base += bd) {
+      const int c = base + tid;
+      const int v = c < c1 ? p.count[c] : 0;
+      int tot;
+      const int ex = block_excl_scan(v, wsum, tot);
+      if (c < c1) {
+        p.start[c] = off + ex;
+        p.cursor[c] = off + ex;
+        p.count[c] = 0;
+      }
+      off += tot;
+    }
+    if (c1 == p.ncells && tid == 0) {
+      p.start[p.ncells] = off;
+      p.cursor[p.ncells] = off;
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) flags[S] = 0;
+  if (p.bin_only) return;
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  // 3. scatter atoms to their cells (order inside a cell fixed in 4)
+  for (int a = gtid; a < p.N; a += gsz)
+    if (p.flags[p.sysid[a]]) p.tmp[atomicAdd(&p.cursor[p.cell[a]], 1)] = a;
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  // 4. stable order inside each cell: rank by original index, one wave per cell (the
+  //    cell's atoms in registers, compared through shuffles)
+  {
+    const int lane = tid & 63;
+    const int gw = gtid >> 6, nw = gsz >> 6;
+    for (int c = gw; c < p.ncells; c += nw) {
+      if (!p.flags[p.cellsys[c]]) continue;
+      const int a = p.start[c], len = p.start[c + 1] - a;
+      if (len <= 64) {
+        const int v = lane < len ? p.tmp[a + lane] : 0x7fffffff;
+        int r = 0;
+        for (int x = 0; x < len; ++x) r += (__shfl(v, x, 64) < v);
+        if (lane < len) p.sorted[a + r] = v;
+      } else {
+        for (int e = lane; e < len; e += 64) {
+          const int v = p.tmp[a + e];
+          int r = 0;
+          for (int x = 0; x < len; ++x) r += (p.tmp[a + x] < v);
+          p.sorted[a + r] = v;
+        }
+      }
+    }
+  }
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  // 5. gather the state into sorted order
+  for (int t = gtid; t < p.N; t += gsz) {
+    const int si = p.sysid[t];
+    if (!p.flags[si]) continue;
+    const int i = p.sorted[t];
+    p.pos2[t] = (rb_par(p, si) ? p.posB : p.posA)[i];
+    p.vel2[t] = p.vel[i];
+    p.F2[t] = p.F[i];
+    p.perm2[t] = p.perm[i];
+    p.cell2[t] = p.cell[i];
+  }
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  // 6. back into the live arrays; the list reference positions are the sorted ones
+  for (int t = gtid; t < p.N; t += gsz) {
+    const int si = p.sysid[t];
+    if (!p.flags[si]) continue;
+    (rb_par(p, si) ? p.posB : p.posA)[t] = p.pos2[t];
+    p.ref[t] = p.pos2[t];
+    p.vel[t] = p.vel2[t];
+    p.F[t] = p.F2[t];
+    p.perm[t] = p.perm2[t];
+    p.cell[t] = p.cell2[t];
+  }
+  if (!grid_sync(p.bar, p.ints + 2)) return;
+  // 7. bookkeeping: parity flips commit, flags move to the list launch, rebuilt systems
+  //    count one rebuild
+  if (blockIdx.x == 0) {
+    for (int s = tid; s < p.S; s += bd) {
+      if (s < p.flip_n) p.par[s] ^= 1;
+      const int f = p.flags[s];
+      p.bflags[s] = f;
+      if (f) {
+        p.nreb[s] += 1;
+        p.flags[s] = 0;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      p.bflags[p.S] = 1;
+      p.flags[p.S] = 0;
+    }
+  }
+}
+
+// Verlet lists of the bricks of the systems the rebuild launch handed over.
+__global__ void __launch_bounds__(kBT) k_b_build(RB p) {
+  if (p.bflags[p.S] == 0 || p.ints[2] != 0) return;
+  const BBrick b = p.bricks[blockIdx.x];
+  if (!p.bflags[b.sys]) return;
+  __shared__ int rs[kMaxRegionCells + 1];
+  extern __shared__ __align__(16) float4 lds32[];
+  int mymax = 0;
+  build_brick(p, b, lds32, rs, mymax);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
+  if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&p.ints[0], mymax);
+}
+
+// Per-system record: sum the bricks' partials in brick order and append to the log.
+__global__ void k_b_record(const BSys* __restrict__ sys, const double* __restrict__ partials,
+                           double* __restrict__ log, int log_cap, int* __restrict__ log_count,
+                           const int* __restrict__ nreb) {
+  const int si = blockIdx.x;
+  const BSys& s = sys[si];
+  __shared__ double red[5 * 8];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
+  block_sum<5>(acc, red);
+  if (threadIdx.x == 0) {
+    const int idx = log_count[si];
+    if (idx < log_cap) {
+      double* o = log + ((size_t)si * log_cap + idx) * kRecW;
+      for (int k = 0; k < 5; ++k) o[k] = acc[k];
+      o[5] = (double)nreb[si];
+      o[6] = 0.0;
+      o[7] = 0.0;
+    }
+    log_count[si] = idx + 1;
+  }
+}
+
+// a[0..n) = v (flags of every system at set-up, rebuild counts)
+__global__ void k_b_fill(int* a, int n, int v) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) a[k] = v;
 }
 
 // ------------------------------------------------------------------ fused step
@@ -366,17 +553,20 @@ __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P q
   }
 }
 
+// One fused (sub)step of one brick: closing half kick with the new forces, then (for
+// bricks below drift_from) the next opening kick and drift into the other position
+// buffer, and the Verlet-distance test; with record, the brick's energy and momentum
+// partials.
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
                                                  const BRegion* __restrict__ reg, const int* __restrict__ start,
                                                  double4* __restrict__ posA, double4* __restrict__ posB,
                                                  const int* __restrict__ par, double4* __restrict__ vel,
-                                                 double4* __restrict__ Fst,
-                                                 const double4* __restrict__ ref, const int* __restrict__ cell,
-                                                 const int* __restrict__ nn, const unsigned short* __restrict__ nbr,
-                                                 int stride, int cap, double wc, double wn, int drift_from,
-                                                 int record, int* __restrict__ flags, int S,
-                                                 double* __restrict__ partials, int* __restrict__ overflow) {
+                                                 double4* __restrict__ Fst, const double4* __restrict__ ref,
+                                                 const int* __restrict__ cell, const int* __restrict__ nn,
+                                                 const uint4* __restrict__ nbr, int stride, int cap, double wc,
+                                                 double wn, int drift_from, int record, int* __restrict__ flags,
+                                                 int S, double* __restrict__ partials, int* __restrict__ ints) {
   typedef typename P4<P>::T T4;
   const BBrick b = bricks[blockIdx.x];
   const BSys& s = sys[b.sys];
@@ -385,8 +575,9 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   __shared__ double red[5 * 8];
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T4* lds = reinterpret_cast<T4*>(lds_raw);
-  const double4* __restrict__ pos = par[b.sys] ? posB : posA;
-  double4* __restrict__ pos_next = par[b.sys] ? posA : posB;
+  const int si = b.sys;
+  const double4* __restrict__ pos = par[si] ? posB : posA;
+  double4* __restrict__ pos_next = par[si] ? posA : posB;
   if constexpr (LJ_ON) {
     const int m = s.ns * s.ns;
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
@@ -397,95 +588,109 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
     if (threadIdx.x == 0) ljt[192] = (P)s.lj_shift;
   }
   const int total = region_starts(b, reg, start, rs);
-  if (total > cap) {  // uniform: the whole block leaves; the host reports the overflow
-    if (threadIdx.x == 0) atomicMax(overflow, total);
-    return;
-  }
-  stage_region<P>(b, s, reg, start, rs, total, pos, lds);
-  __syncthreads();
-  const double h_cur = 0.5 * wc * s.dt, h_next = 0.5 * wn * s.dt, dt_next = wn * s.dt;
-  const bool do_drift = (int)blockIdx.x < drift_from;
-  const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
+  // uniform: an overflowing region (or a failed rebuild) skips the work; the host reports it
+  const bool ok = total + 1 <= cap && ints[2] == 0;
+  if (!ok && threadIdx.x == 0 && total + 1 > cap) atomicMax(&ints[1], total + 1);
   double acc[5] = {0, 0, 0, 0, 0};
   bool moved_far = false;
-  for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-    const int local = cell[a] - b.cell0;
-    const int self = rs[home_region(b, local)] + (a - start[b.cell0 + local]);
-    const T4 xi = lds[self];
-    const int ti = ONE_SP ? 0 : (int)xi.w;
-    const int row = ti * s.ns;
-    const P qi = (P)s.charge[ti];
-    const int cnt = nn[a];
-    double Fx = 0, Fy = 0, Fz = 0, U = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
-    P fx = 0, fy = 0, fz = 0, u = 0, cx = 0, cy = 0, cz = 0, cu = 0;
-    int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-      const int l0 = nbr[(size_t)k * stride + a], l1 = nbr[(size_t)(k + 1) * stride + a];
-      const int l2 = nbr[(size_t)(k + 2) * stride + a], l3 = nbr[(size_t)(k + 3) * stride + a];
-      const T4 x0 = lds[l0], x1 = lds[l1], x2 = lds[l2], x3 = lds[l3];
-      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, cx, cy, cz, cu);
-      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, cx, cy, cz, cu);
-      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, cx, cy, cz, cu);
-      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, cx, cy, cz, cu);
-      if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 4 neighbours
-        Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
-        fx = fy = fz = u = cx = cy = cz = cu = 0;
+  if (ok) {
+    stage_region<P>(b, s, reg, start, rs, total, pos, lds);
+    if (threadIdx.x == 0) {  // far sentinel: the padding entries of the lists point here
+      T4 far;
+      far.x = (P)1e15;
+      far.y = (P)1e15;
+      far.z = (P)1e15;
+      far.w = (P)0;
+      lds[total] = far;
+    }
+    __syncthreads();
+    const double h_cur = 0.5 * wc * s.dt, h_next = 0.5 * wn * s.dt, dt_next = wn * s.dt;
+    const bool do_drift = (int)blockIdx.x < drift_from;
+    const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
+    for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+      const int local = cell[a] - b.cell0;
+      const int self = rs[home_region(b, local)] + (a - start[b.cell0 + local]);
+      const T4 xi = lds[self];
+      const int ti = ONE_SP ? 0 : (int)xi.w;
+      const int row = ti * s.ns;
+      const P qi = (P)s.charge[ti];
+      const int ng = (nn[a] + 7) >> 3;
+      double Fx = 0, Fy = 0, Fz = 0, U = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
+      P fx = 0, fy = 0, fz = 0, u = 0, cx = 0, cy = 0, cz = 0, cu = 0;
+      // groups of 8 neighbours; the next group's indices load while this one computes
+      uint4 cur = make_uint4(0u, 0u, 0u, 0u);
+      if (ng > 0) cur = nbr[a];
+      for (int g = 0; g < ng; ++g) {
+        uint4 nxt = cur;
+        if (g + 1 < ng) nxt = nbr[(size_t)(g + 1) * stride + a];
+        const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+        for (int h2 = 0; h2 < 2; ++h2) {
+          const T4 x0 = lds[w[2 * h2] & 0xffffu], x1 = lds[w[2 * h2] >> 16];
+          const T4 x2 = lds[w[2 * h2 + 1] & 0xffffu], x3 = lds[w[2 * h2 + 1] >> 16];
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, cx, cy, cz, cu);
+          if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 4 neighbours
+            Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+            fx = fy = fz = u = cx = cy = cz = cu = 0;
+          }
+        }
+        cur = nxt;
       }
-    }
-    for (; k < cnt; ++k)
-      b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, lds[nbr[(size_t)k * stride + a]], fx, fy, fz, u, cx, cy,
-                                        cz, cu);
-    Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
-    if constexpr (COUL_ON) {
-      const double q = s.charge[ti];
-      CU -= q * q * s.coul.self;
-      Fx += s.coul.k * Cx;
-      Fy += s.coul.k * Cy;
-      Fz += s.coul.k * Cz;
-      U += s.coul.k * CU;
-    }
-    const double m = s.mass[ti];
-    const double ax = Fx / m, ay = Fy / m, az = Fz / m;
-    double4 v = vel[a];
-    {
-#pragma clang fp contract(off)
-      v.x = v.x + h_cur * ax;
-      v.y = v.y + h_cur * ay;
-      v.z = v.z + h_cur * az;
-    }
-    if (record) {
-      acc[0] += U;
-      acc[1] += 0.5 * m * ((v.x * v.x + v.z * v.z) + v.y * v.y);
-      acc[2] += m * v.x;
-      acc[3] += m * v.y;
-      acc[4] += m * v.z;
-    }
-    Fst[a] = make_double4(Fx, Fy, Fz, 0.0);
-    if (do_drift) {
-      double4 x = pos[a];
+      Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+      if constexpr (COUL_ON) {
+        const double q = s.charge[ti];
+        CU -= q * q * s.coul.self;
+        Fx += s.coul.k * Cx;
+        Fy += s.coul.k * Cy;
+        Fz += s.coul.k * Cz;
+        U += s.coul.k * CU;
+      }
+      const double m = s.mass[ti];
+      const double ax = Fx / m, ay = Fy / m, az = Fz / m;
+      double4 v = vel[a];
       {
 #pragma clang fp contract(off)
-        v.x = v.x + h_next * ax;
-        v.y = v.y + h_next * ay;
-        v.z = v.z + h_next * az;
-        x.x = x.x + dt_next * v.x;
-        x.y = x.y + dt_next * v.y;
-        x.z = x.z + dt_next * v.z;
+        v.x = v.x + h_cur * ax;
+        v.y = v.y + h_cur * ay;
+        v.z = v.z + h_cur * az;
       }
-      x.x = np_mod2(x.x, s.L[0]);
-      x.y = np_mod2(x.y, s.L[1]);
-      x.z = np_mod2(x.z, s.L[2]);
-      pos_next[a] = x;
-      const double4 r = ref[a];
-      const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
-      const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
-      const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
-      moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+      if (record) {
+        acc[0] += U;
+        acc[1] += 0.5 * m * ((v.x * v.x + v.z * v.z) + v.y * v.y);
+        acc[2] += m * v.x;
+        acc[3] += m * v.y;
+        acc[4] += m * v.z;
+      }
+      Fst[a] = make_double4(Fx, Fy, Fz, 0.0);
+      if (do_drift) {
+        double4 x = pos[a];
+        {
+#pragma clang fp contract(off)
+          v.x = v.x + h_next * ax;
+          v.y = v.y + h_next * ay;
+          v.z = v.z + h_next * az;
+          x.x = x.x + dt_next * v.x;
+          x.y = x.y + dt_next * v.y;
+          x.z = x.z + dt_next * v.z;
+        }
+        x.x = np_mod2(x.x, s.L[0]);
+        x.y = np_mod2(x.y, s.L[1]);
+        x.z = np_mod2(x.z, s.L[2]);
+        pos_next[a] = x;
+        const double4 r = ref[a];
+        const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+        const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+        const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+        moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+      }
+      vel[a] = v;
     }
-    vel[a] = v;
   }
   if (__any(moved_far) && (threadIdx.x & 63) == 0) {
-    atomicOr(&flags[b.sys], 1);
+    atomicOr(&flags[si], 1);
     atomicOr(&flags[S], 1);
   }
   if (record) {
@@ -539,38 +744,6 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
   }
 }
 
-// Per-system record: sum the bricks' partials in brick order and append to the log.
-__global__ void k_b_record(const BSys* __restrict__ sys, const double* __restrict__ partials,
-                           double* __restrict__ log, int log_cap, int* __restrict__ log_count,
-                           const int* __restrict__ nreb) {
-  const int si = blockIdx.x;
-  const BSys& s = sys[si];
-  __shared__ double red[5 * 8];
-  double acc[5] = {0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
-    for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
-  block_sum<5>(acc, red);
-  if (threadIdx.x == 0) {
-    const int idx = log_count[si];
-    if (idx < log_cap) {
-      double* o = log + ((size_t)si * log_cap + idx) * kRecW;
-      for (int k = 0; k < 5; ++k) o[k] = acc[k];
-      o[5] = (double)nreb[si];
-      o[6] = 0.0;
-      o[7] = 0.0;
-    }
-    log_count[si] = idx + 1;
-  }
-}
-
-__global__ void k_b_set_all(int* flags, int S) {
-  for (int s = threadIdx.x; s <= S; s += blockDim.x) flags[s] = 1;
-}
-
-__global__ void k_b_flip(int* __restrict__ par, int na) {
-  for (int s = threadIdx.x; s < na; s += blockDim.x) par[s] ^= 1;
-}
-
 // a (or, with b != NULL, the buffer par selects per system) scattered to user order
 __global__ void k_b_scatter_user(int natoms, const int* __restrict__ perm, const double4* __restrict__ a,
                                  const double4* __restrict__ b, const int* __restrict__ par,
@@ -619,11 +792,16 @@ struct BMD {
   int *perm = nullptr, *perm2 = nullptr, *cell = nullptr, *cell2 = nullptr;
   int *count = nullptr, *start = nullptr, *cursor = nullptr, *tmp = nullptr, *sorted = nullptr;
   int* nn = nullptr;
-  unsigned short* nbr = nullptr;
-  int stride = 0, max_nbrs = 0;
+  uint4* nbr = nullptr;  // groups of 8 u16 local indices, group g of atom a at [g * stride + a]
+  int stride = 0, max_nbrs = 0;  // max_nbrs: list capacity per atom, a multiple of 8
   int* flags = nullptr;  // [S] per system + [S] any
-  int* ints = nullptr;   // [0] max neighbours, [1] region overflow
+  int* ints = nullptr;   // [0] max neighbours, [1] region overflow, [2] rebuild barrier timeout
   int* nreb = nullptr;
+  int* bflags = nullptr;   // rebuild flags handed to the list launch
+  unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
+  int* bsum = nullptr;      // per-block sums of the rebuild scan
+  static constexpr int kMaxGrid = 1 << 16;
+  int grid_rb = 0;  // resident grid of the rebuild launch
   int* log_count = nullptr;
   double* log = nullptr;
   int log_cap = 0;
@@ -641,7 +819,7 @@ struct BMD {
   void release() {
     void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, vel, F, ref, pos2, vel2, F2,
                   perm, perm2, cell, cell2, count, start, cursor, tmp, sorted, nn, nbr, flags, ints, nreb, log_count,
-                  log, partials};
+                  log, partials, bflags, bar, bsum};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -653,44 +831,95 @@ struct BMD {
   size_t lds_fused() const { return (size_t)cap_fused * (precision == ACCU_FP64 ? sizeof(double4) : sizeof(float4)); }
   size_t lds_build() const { return (size_t)cap_build * sizeof(float4); }
 
-  void launch_rebuild() {
-    const int T = 256;
-    hipLaunchKernelGGL(k_b_zero, dim3((ncells + T - 1) / T), dim3(T), 0, st, d_cellsys, count, ncells, flags, S);
-    hipLaunchKernelGGL(k_b_assign, dim3((N + T - 1) / T), dim3(T), 0, st, pos, pos_b, par, d_sysid, d_sys, d_cellmap,
-                       cell, count, N, flags, S);
-    hipLaunchKernelGGL(k_b_scan, dim3(1), dim3(1024), 0, st, count, start, cursor, ncells, flags, S);
-    hipLaunchKernelGGL(k_b_scatter, dim3((N + T - 1) / T), dim3(T), 0, st, cell, d_sysid, N, cursor, tmp, flags, S);
-    hipLaunchKernelGGL(k_b_sort, dim3((ncells + 3) / 4), dim3(256), 0, st, start, d_cellsys, ncells, tmp, sorted, flags,
-                       S);
-    hipLaunchKernelGGL(k_b_permute, dim3((N + T - 1) / T), dim3(T), 0, st, N, d_sysid, sorted, pos, pos_b, par, vel, F,
-                       perm, cell, pos2, vel2, F2, perm2, cell2, flags, S);
-    hipLaunchKernelGGL(k_b_copyback, dim3((N + T - 1) / T), dim3(T), 0, st, N, d_sysid, pos, pos_b, par, ref, vel, F,
-                       perm, cell, pos2, vel2, F2, perm2, cell2, flags, S);
-    hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(threads), lds_build(), st, d_sys, d_bricks, d_reg, start, pos, pos_b,
-                       par, cell, nn, nbr, stride, max_nbrs, cap_build, ints, ints + 1, flags, S);
-    hipLaunchKernelGGL(k_b_end, dim3(1), dim3(256), 0, st, flags, S, nreb);
+  RB rebuild_args(int bin_only, int flip_n) const {
+    RB p;
+    p.sys = d_sys;
+    p.bricks = d_bricks;
+    p.reg = d_reg;
+    p.cellmap = d_cellmap;
+    p.cellsys = d_cellsys;
+    p.sysid = d_sysid;
+    p.posA = pos;
+    p.posB = pos_b;
+    p.par = par;
+    p.ref = ref;
+    p.vel = vel;
+    p.F = F;
+    p.perm = perm;
+    p.cell = cell;
+    p.pos2 = pos2;
+    p.vel2 = vel2;
+    p.F2 = F2;
+    p.perm2 = perm2;
+    p.cell2 = cell2;
+    p.count = count;
+    p.start = start;
+    p.cursor = cursor;
+    p.tmp = tmp;
+    p.sorted = sorted;
+    p.bsum = bsum;
+    p.nn = nn;
+    p.nbr = nbr;
+    p.stride = stride;
+    p.ngroups = max_nbrs / 8;
+    p.cap = cap_build;
+    p.ints = ints;
+    p.bar = bar;
+    p.flags = flags;
+    p.bflags = bflags;
+    p.nreb = nreb;
+    p.flip_n = flip_n;
+    p.S = S;
+    p.N = N;
+    p.ncells = ncells;
+    p.nbricks = nbricks;
+    p.bin_only = bin_only;
+    return p;
+  }
+
+  // Grid of the rebuild launch: one block of kRB threads per CU, all resident at once
+  // (16 waves of a CU's 28 or more; the query must allow one block).
+  int resident_grid() {
+    int per = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_b_rebuild, kRB, 0) != hipSuccess)
+      return 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    if (per < 1) return 0;
+    return std::min(ncu, kMaxGrid);
+  }
+
+  // Rebuild of the flagged systems (binning, sort, lists); systems [0, flip_n) drifted
+  // in the step before and flip their buffer parity.
+  void launch_rebuild(int flip_n) {
+    const RB p = rebuild_args(0, flip_n);
+    hipLaunchKernelGGL(k_b_rebuild, dim3(grid_rb), dim3(kRB), 0, st, p);
+    hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(threads), lds_build(), st, p);
   }
 
   template <typename P, bool ONE>
   void fused_t(int nb, double wc, double wn, int drift_from, int rec, bool lj, bool co) {
     const size_t sh = lds_fused();
+
     if (lj && co)
       hipLaunchKernelGGL((k_b_fused<P, true, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
                          pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
-                         flags, S, partials, ints + 1);
+                         flags, S, partials, ints);
     else if (lj)
       hipLaunchKernelGGL((k_b_fused<P, true, false, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
                          pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
-                         flags, S, partials, ints + 1);
+                         flags, S, partials, ints);
     else
       hipLaunchKernelGGL((k_b_fused<P, false, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
                          pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
-                         flags, S, partials, ints + 1);
+                         flags, S, partials, ints);
   }
 
   bool any_lj = false, any_coul = false, one_sp = true;
 
-  // One fused (sub)step of the first `nb` bricks; bricks below drift_from also drift.
+  // One fused (sub)step of the first `nb` bricks; bricks below drift_from also drift
+  // (their systems' buffer parity flips with the rebuild launch that follows).
   void launch_fused(int nb, double wc, double wn, int drift_from, int rec) {
     if (timing) {
       if (ev_used + 2 > ev.size()) {
@@ -719,13 +948,12 @@ struct BMD {
   // Enqueue `coarse` coarse steps of `nticks` ticks; at tick t the first active[t]
   // systems advance one (sub)step with weights wc[t] (closing kick) and wn[t] (next
   // opening kick and drift). The last tick must advance every system. Each system's
-  // buffer parity flips with every drift, on the device.
+  // buffer parity flips with every drift, on the device (inside the rebuild launch).
   void enqueue(int coarse, int nticks, const int* active, const double* wc, const double* wn, int record_every) {
     const int T = 256;
     hipLaunchKernelGGL(k_b_drift, dim3((N + T - 1) / T), dim3(T), 0, st, d_sys, d_sysid, N, pos, pos_b, par, vel, F,
                        ref, wc[0], flags, S);
-    hipLaunchKernelGGL(k_b_flip, dim3(1), dim3(256), 0, st, par, S);
-    launch_rebuild();
+    launch_rebuild(S);
     for (int c = 0; c < coarse; ++c) {
       for (int t = 0; t < nticks; ++t) {
         const int na = active[t];
@@ -733,10 +961,7 @@ struct BMD {
         const bool last = (c == coarse - 1) && (t == nticks - 1);
         const int rec = (t == nticks - 1) && record_every > 0 && ((c + 1) % record_every == 0 || c == coarse - 1);
         launch_fused(nb, wc[t], wn[t], last ? 0 : nb, rec);
-        if (!last) {
-          hipLaunchKernelGGL(k_b_flip, dim3(1), dim3(256), 0, st, par, na);
-          launch_rebuild();
-        }
+        if (!last) launch_rebuild(na);
       }
     }
   }
@@ -1017,18 +1242,27 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   m.log_cap = std::max(log_cap, 4);
   HIP_TRY(hipMalloc(&m.log, (size_t)m.S * m.log_cap * kRecW * sizeof(double)));
   m.stride = (m.N + 63) / 64 * 64;
+  HIP_TRY(hipMemset(m.count, 0, (m.ncells + 1) * sizeof(int)));
+  HIP_TRY(hipMalloc(&m.bflags, (m.S + 1) * sizeof(int) + 64));
+  HIP_TRY(hipMemset(m.bflags, 0, (m.S + 1) * sizeof(int) + 64));
+  HIP_TRY(hipMalloc(&m.bar, 64));
+  HIP_TRY(hipMemset(m.bar, 0, 64));
+  HIP_TRY(hipMalloc(&m.bsum, BMD::kMaxGrid * sizeof(int)));
   // first binning of every system: measure the region sizes to size the LDS
-  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
-  const int T = 256;
-  hipLaunchKernelGGL(k_b_zero, dim3((m.ncells + T - 1) / T), dim3(T), 0, m.st, m.d_cellsys, m.count, m.ncells, m.flags,
-                     m.S);
-  hipLaunchKernelGGL(k_b_assign, dim3((m.N + T - 1) / T), dim3(T), 0, m.st, m.pos, m.pos_b, m.par, m.d_sysid, m.d_sys,
-                     m.d_cellmap, m.cell, m.count, m.N, m.flags, m.S);
-  hipLaunchKernelGGL(k_b_scan, dim3(1), dim3(1024), 0, m.st, m.count, m.start, m.cursor, m.ncells, m.flags, m.S);
+  m.grid_rb = m.resident_grid();
+  ARG_CHECK(m.grid_rb > 0, "the rebuild launch (%d threads per block) does not fit on a CU", kRB);
+  hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, m.st, m.flags, m.S + 1, 1);
+  hipLaunchKernelGGL(k_b_rebuild, dim3(m.grid_rb), dim3(kRB), 0, m.st, m.rebuild_args(1, 0));
   LAUNCH_CHECK();
   std::vector<int> hstart(m.ncells + 1);
   HIP_TRY(hipMemcpyAsync(hstart.data(), m.start, (m.ncells + 1) * sizeof(int), hipMemcpyDeviceToHost, m.st));
-  HIP_TRY(hipStreamSynchronize(m.st));
+  {
+    int e[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(e, m.ints, sizeof e, hipMemcpyDeviceToHost, m.st));
+    HIP_TRY(hipStreamSynchronize(m.st));
+    ARG_CHECK(e[2] == 0, "rebuild grid barrier timed out at the first binning: %d of %d blocks arrived", e[3],
+              m.grid_rb);
+  }
   std::vector<BBrick> hb(m.nbricks);
   HIP_TRY(hipMemcpy(hb.data(), m.d_bricks, m.nbricks * sizeof(BBrick), hipMemcpyDeviceToHost));
   int regmax = 0;
@@ -1047,7 +1281,7 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
       regmax = std::max(regmax, tot);
     }
   }
-  const int cap = (int)(regmax * (1.0 + slack)) + 64;
+  const int cap = (int)(regmax * (1.0 + slack)) + 64;  // + 1 sentinel slot included
   hipDeviceProp_t prop;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -1055,28 +1289,27 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   const size_t lds_max = prop.sharedMemPerBlock;
   m.cap_build = cap;
   m.cap_fused = cap;
-  ARG_CHECK(m.lds_build() + 4096 <= lds_max && cap < 65536,
+  ARG_CHECK(m.lds_build() + 4096 <= lds_max && m.lds_fused() + 4096 <= lds_max && cap < 65536,
             "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
   (void)hipFuncSetAttribute((const void*)k_b_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
-  // neighbour capacity: measure with a small capacity, then size
-  m.max_nbrs = 1;
-  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * sizeof(unsigned short)));
-  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
-  m.launch_rebuild();
-  int ints[2] = {0, 0};
+  // neighbour capacity: measure with one group of 8, then size
+  m.max_nbrs = 8;
+  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * sizeof(uint4)));
+  hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, m.st, m.flags, m.S + 1, 1);
+  m.launch_rebuild(0);
+  int ints[4] = {0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(ints, m.ints, sizeof ints, hipMemcpyDeviceToHost, m.st));
   HIP_TRY(hipStreamSynchronize(m.st));
+  ARG_CHECK(ints[2] == 0, "rebuild grid barrier timed out at set-up: %d of %d blocks arrived", ints[3], m.grid_rb);
   ARG_CHECK(ints[1] == 0, "region overflow at the first build (%d atoms)", ints[1]);
-  const int nbrs = (int)(ints[0] * (1.0 + slack)) + 16;
+  const int nbrs = ((int)(ints[0] * (1.0 + slack)) + 16 + 7) / 8 * 8;
   HIP_TRY(hipFree(m.nbr));
-  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * nbrs * sizeof(unsigned short)));
+  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * (nbrs / 8) * sizeof(uint4)));
   m.max_nbrs = nbrs;
-  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
-  m.launch_rebuild();
-  // the two measuring builds count as one rebuild per system
-  HIP_TRY(hipMemsetAsync(m.nreb, 0, m.S * sizeof(int), m.st));
-  hipLaunchKernelGGL(k_b_set_all, dim3(1), dim3(256), 0, m.st, m.flags, m.S);
-  hipLaunchKernelGGL(k_b_end, dim3(1), dim3(256), 0, m.st, m.flags, m.S, m.nreb);
+  hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, m.st, m.flags, m.S + 1, 1);
+  m.launch_rebuild(0);
+  // the measuring builds count as one rebuild per system
+  hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, m.st, m.nreb, m.S, 1);
   // forces at t = 0, first record
   auto set_attr = [&](const void* f) {
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_fused());
@@ -1173,7 +1406,7 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
     HIP_TRY(hipMemcpyAsync(dsts[k], tmp, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, m.st));
   }
   std::vector<int> nreb(m.S), lc(m.S);
-  int ints[2];
+  int ints[4];
   HIP_TRY(hipMemcpyAsync(nreb.data(), m.nreb, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
   HIP_TRY(hipMemcpyAsync(lc.data(), m.log_count, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
   HIP_TRY(hipMemcpyAsync(ints, m.ints, sizeof ints, hipMemcpyDeviceToHost, m.st));
@@ -1205,6 +1438,10 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
     long long tot = 0;
     for (int v : nn) tot += v;
     info[4] = tot;
+  }
+  if (ints[2] != 0) {
+    set_error("rebuild grid barrier timed out: %d of %d blocks arrived", ints[3], m.grid_rb);
+    return ACCU_ERUNTIME;
   }
   if (ints[1] > 0) {
     set_error("brick region overflow: %d atoms > LDS capacity %d (raise the slack or use smaller bricks)", ints[1],
