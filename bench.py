@@ -14,7 +14,9 @@ the 65,536-ion NaCl LJ + DSF workload and the learned-correction throughput.
 roofline: the fused force + integrate kernel. Algorithmic bytes per launch
   = N * 148 B (read x, v, F and the type; write x, v, F) + 4 B per half-list pair
   within rc + skin, divided by its mean launch time from HIP events on the engine's
-  stream over the timed region; peak 8 TB/s (HBM3E).
+  stream over the timed region; peak 8 TB/s (HBM3E). traffic: the HBM bytes per
+  launch that rocprofv3's FETCH_SIZE / WRITE_SIZE counters measure (calibrated on a
+  plain copy).
 cpu_baseline: the NumPy float64 oracle (one thread) on a 131,072-atom sample of the
   same density (5 VV steps after one list build), scaled to 1,048,576 atoms by N.
 """
@@ -32,6 +34,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the mixed fused kernel on this workload (bricks and threads
+# below): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors being
+# what the two counters read for a 1 GiB device copy (bench.py --copy-probe 1024); the
+# summary with its commands is profiles/k_b_fused_lj_argon_1048576.md
+FUSED_TRAFFIC_BYTES = 5.793e8
 DT = 10e-15
 # tile choice of the autotuner on MI355X for this workload (devinproj_amd/autotune.py)
 LJ_BRICKS = (2, 2, 3)
@@ -239,7 +246,7 @@ def main():
         "nacl_65536_lj_dsf_steps_per_s": {k: v["steps_per_s"] for k, v in nacl.items()} if nacl else None,
         "learned_dsf_vs_ewald": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": FUSED_TRAFFIC_BYTES,
                      "kernel": "k_b_fused<float,LJ>", "kernel_ms": mixed["kernel_ms"],
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,

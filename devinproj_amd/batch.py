@@ -39,7 +39,15 @@ def default_bricks(s, rc_list, cap_atoms=1500):
 
 
 class BrickEngine:
-    def __init__(self, systems, forces, integrator=None, dts=None, precision="fp64", bricks=None, slack=0.4,
+    """Device-resident MD of one or many periodic systems (csrc/brick.hip).
+
+    `slack` is the relative headroom of the neighbour-list and LDS region capacities
+    over what the first build measures. A region or list that outgrows it later is
+    reported as an error at fetch(). The LDS region sets how many workgroups share a CU,
+    so a larger slack costs occupancy.
+    """
+
+    def __init__(self, systems, forces, integrator=None, dts=None, precision="fp64", bricks=None, slack=0.25,
                  log_cap=20_000, threads=256):
         self.systems = list(systems)
         S = len(self.systems)

@@ -179,7 +179,41 @@ struct RB {
   int S, N, ncells, nbricks;
   int flip_n;    // systems [0, flip_n) drifted: their parity flips with this launch
   int bin_only;  // 1: stop after the scan (set-up measures the regions)
+  // record of the step before (few systems): block 0 sums the brick partials
+  int rec;
+  const double* partials;
+  double* log;
+  int log_cap;
+  int* log_count;
 };
+
+constexpr int kRecInline = 4;  // up to this many systems the rebuild launch records; else k_b_record
+
+// Per-system sums of the brick partials in brick order (fixed thread assignment and a
+// fixed-order block sum), appended to the system's log with its rebuild count.
+__device__ void record_systems(const BSys* __restrict__ sys, int S, const double* __restrict__ partials,
+                               double* __restrict__ log, int log_cap, int* __restrict__ log_count,
+                               const int* __restrict__ nreb, double* red) {
+  for (int si = 0; si < S; ++si) {
+    const BSys& s = sys[si];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
+    block_sum<5>(acc, red);
+    if (threadIdx.x == 0) {
+      const int idx = log_count[si];
+      if (idx < log_cap) {
+        double* o = log + ((size_t)si * log_cap + idx) * kRecW;
+        for (int k = 0; k < 5; ++k) o[k] = acc[k];
+        o[5] = (double)nreb[si];
+        o[6] = 0.0;
+        o[7] = 0.0;
+      }
+      log_count[si] = idx + 1;
+    }
+  }
+}
 
 // Grid-wide barrier. Returns false when the wait times out: err[0] is then set (every
 // block leaves and the engine reports the error at the next get) and err[1] keeps the
@@ -337,6 +371,10 @@ __device__ __forceinline__ int rb_par(const RB& p, int si) { return p.par[si] ^ 
 
 __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   const int tid = threadIdx.x, bd = blockDim.x;
+  if (p.rec && blockIdx.x == 0) {
+    __shared__ double red[5 * (kRB / 64)];
+    record_systems(p.sys, p.S, p.partials, p.log, p.log_cap, p.log_count, p.nreb, red);
+  }
   if (p.flags[p.S] == 0 || p.ints[2] != 0) {
     if (blockIdx.x == 0) {  // nothing to rebuild: commit the flip, the list launch exits
       for (int s = tid; s < p.flip_n; s += bd) p.par[s] ^= 1;
@@ -486,29 +524,13 @@ __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&p.ints[0], mymax);
 }
 
-// Per-system record: sum the bricks' partials in brick order and append to the log.
+// Per-system record, one block per system (many systems).
 __global__ void k_b_record(const BSys* __restrict__ sys, const double* __restrict__ partials,
                            double* __restrict__ log, int log_cap, int* __restrict__ log_count,
                            const int* __restrict__ nreb) {
-  const int si = blockIdx.x;
-  const BSys& s = sys[si];
   __shared__ double red[5 * 8];
-  double acc[5] = {0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
-  block_sum<5>(acc, red);
-  if (threadIdx.x == 0) {
-    const int idx = log_count[si];
-    if (idx < log_cap) {
-      double* o = log + ((size_t)si * log_cap + idx) * kRecW;
-      for (int k = 0; k < 5; ++k) o[k] = acc[k];
-      o[5] = (double)nreb[si];
-      o[6] = 0.0;
-      o[7] = 0.0;
-    }
-    log_count[si] = idx + 1;
-  }
+  const int si = blockIdx.x;
+  record_systems(sys + si, 1, partials, log + (size_t)si * log_cap * kRecW, log_cap, log_count + si, nreb + si, red);
 }
 
 // a[0..n) = v (flags of every system at set-up, rebuild counts)
@@ -528,24 +550,56 @@ __device__ __forceinline__ double np_mod2(double x, double L) {
   return r;
 }
 
+// LJ pair of the engine (forces.py:127-141, full list so half the energy per visit):
+//   a = sr6^2, b = a - sr6; u += 2 eps (b - shift); coef = -24 eps (a + b) / r2.
+// The cutoff is an arithmetic 0/1 factor rather than a branch, so the eight pairs of a
+// group run as straight-line code: in = clamp((rc2 - r2) * ks, 0, 1) with
+// ks = 2^40 / rc2 (float32; 2^80 in float64) is exactly 1 for every r2 < rc2 (the
+// smallest nonzero relative gap is one ulp, 2^-24 or 2^-53) and exactly 0 for r2 >= rc2.
+// Tables: e2 = 2 eps, m24 = -24 eps, and rc2 = -1 (ks = 0 is not used: in = 0) where
+// eps = 0. x is capped at 1e6 so no power overflows for pairs the factor removes.
+template <typename P>
+__device__ __forceinline__ void lj_engine(P dx, P dy, P dz, P r2, P e2, P m24, P sig2, P rc2, P ks, P shift, P& fx,
+                                          P& fy, P& fz, P& u) {
+  P ir2, in;
+  if constexpr (sizeof(P) == 8) {
+    ir2 = rcp64(r2);
+    in = fmin(fmax((rc2 - r2) * ks, 0.0), 1.0);
+  } else {
+    ir2 = __builtin_amdgcn_rcpf(r2);
+    in = __builtin_amdgcn_fmed3f((rc2 - r2) * ks, 0.0f, 1.0f);
+  }
+  const P x = fmin(sig2 * ir2, P(1e6));
+  const P sr6 = x * x * x;
+  const P a = sr6 * sr6;
+  const P b = a - sr6;
+  u = fma(e2 * in, b - shift, u);
+  const P coef = (m24 * in) * ((a + b) * ir2);
+  fx = fma(coef, dx, fx);
+  fy = fma(coef, dy, fy);
+  fz = fma(coef, dz, fz);
+}
+
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P qi, const typename P4<P>::T& xi,
                                        const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& cx, P& cy, P& cz,
                                        P& cu) {
   const P dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
   const int tj = ONE_SP ? 0 : (int)xj.w;
+  // list membership is decided at the build, so r^2 can use fused multiply-adds here
+  const P r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   if constexpr (sizeof(P) == 8) {
-    const double r2 = r2_np(dx, dy, dz);
     if constexpr (LJ_ON) {
       const int pj = row + tj;
-      lj_pair64(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192], fx, fy, fz, u);
+      lj_engine<double>(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192 + pj], ljt[256 + pj], ljt[320],
+                        fx, fy, fz, u);
     }
     if constexpr (COUL_ON) coul_pair64(s.coul, dx, dy, dz, r2, qi * s.charge[tj], cx, cy, cz, cu);
   } else {
-    const float r2 = r2_npf(dx, dy, dz);
     if constexpr (LJ_ON) {
       const int pj = row + tj;
-      lj_pair32(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192], fx, fy, fz, u);
+      lj_engine<float>(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192 + pj], ljt[256 + pj], ljt[320],
+                       fx, fy, fz, u);
     }
     if constexpr (COUL_ON)
       coul_pair32((float)s.coul.alpha, (float)s.coul.rc, (float)s.coul.c, (float)s.coul.erc_rc, (float)s.coul.fshift,
@@ -571,7 +625,7 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   const BBrick b = bricks[blockIdx.x];
   const BSys& s = sys[b.sys];
   __shared__ int rs[kMaxRegionCells + 1];
-  __shared__ P ljt[3 * 64 + 1];
+  __shared__ P ljt[5 * 64 + 1];
   __shared__ double red[5 * 8];
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T4* lds = reinterpret_cast<T4*>(lds_raw);
@@ -581,11 +635,16 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   if constexpr (LJ_ON) {
     const int m = s.ns * s.ns;
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
-      ljt[k] = (P)s.lj_eps[k];
-      ljt[64 + k] = (P)s.lj_sig2[k];
-      ljt[128 + k] = (P)s.lj_rc2[k];
+      const double e = s.lj_eps[k];
+      ljt[k] = (P)(2.0 * e);
+      ljt[64 + k] = (P)(-24.0 * e);
+      ljt[128 + k] = (P)s.lj_sig2[k];
+      const bool on = e > 0.0 && s.lj_rc2[k] > 0.0;
+      ljt[192 + k] = on ? (P)s.lj_rc2[k] : (P)-1;
+      // 2^40 / rc2 in float32, 2^80 / rc2 in float64 (see lj_engine)
+      ljt[256 + k] = on ? (P)((sizeof(P) == 8 ? 0x1p80 : 0x1p40) / s.lj_rc2[k]) : (P)0;
     }
-    if (threadIdx.x == 0) ljt[192] = (P)s.lj_shift;
+    if (threadIdx.x == 0) ljt[320] = (P)s.lj_shift;
   }
   const int total = region_starts(b, reg, start, rs);
   // uniform: an overflowing region (or a failed rebuild) skips the work; the host reports it
@@ -632,10 +691,10 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
           b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, cx, cy, cz, cu);
           b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, cx, cy, cz, cu);
           b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, cx, cy, cz, cu);
-          if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 4 neighbours
-            Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
-            fx = fy = fz = u = cx = cy = cz = cu = 0;
-          }
+        }
+        if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 8 neighbours
+          Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+          fx = fy = fz = u = cx = cy = cz = cu = 0;
         }
         cur = nxt;
       }
@@ -831,7 +890,7 @@ struct BMD {
   size_t lds_fused() const { return (size_t)cap_fused * (precision == ACCU_FP64 ? sizeof(double4) : sizeof(float4)); }
   size_t lds_build() const { return (size_t)cap_build * sizeof(float4); }
 
-  RB rebuild_args(int bin_only, int flip_n) const {
+  RB rebuild_args(int bin_only, int flip_n, int rec = 0) const {
     RB p;
     p.sys = d_sys;
     p.bricks = d_bricks;
@@ -869,6 +928,11 @@ struct BMD {
     p.bflags = bflags;
     p.nreb = nreb;
     p.flip_n = flip_n;
+    p.rec = rec;
+    p.partials = partials;
+    p.log = log;
+    p.log_cap = log_cap;
+    p.log_count = log_count;
     p.S = S;
     p.N = N;
     p.ncells = ncells;
@@ -891,9 +955,9 @@ struct BMD {
   }
 
   // Rebuild of the flagged systems (binning, sort, lists); systems [0, flip_n) drifted
-  // in the step before and flip their buffer parity.
-  void launch_rebuild(int flip_n) {
-    const RB p = rebuild_args(0, flip_n);
+  // in the step before and flip their buffer parity; rec: record that step first.
+  void launch_rebuild(int flip_n, int rec = 0) {
+    const RB p = rebuild_args(0, flip_n, rec);
     hipLaunchKernelGGL(k_b_rebuild, dim3(grid_rb), dim3(kRB), 0, st, p);
     hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(threads), lds_build(), st, p);
   }
@@ -942,7 +1006,10 @@ struct BMD {
       (void)hipEventRecord(ev[ev_used + 1], st);
       ev_used += 2;
     }
-    if (rec) hipLaunchKernelGGL(k_b_record, dim3(S), dim3(256), 0, st, d_sys, partials, log, log_cap, log_count, nreb);
+  }
+
+  void launch_record() {
+    hipLaunchKernelGGL(k_b_record, dim3(S), dim3(256), 0, st, d_sys, partials, log, log_cap, log_count, nreb);
   }
 
   // Enqueue `coarse` coarse steps of `nticks` ticks; at tick t the first active[t]
@@ -961,7 +1028,9 @@ struct BMD {
         const bool last = (c == coarse - 1) && (t == nticks - 1);
         const int rec = (t == nticks - 1) && record_every > 0 && ((c + 1) % record_every == 0 || c == coarse - 1);
         launch_fused(nb, wc[t], wn[t], last ? 0 : nb, rec);
-        if (!last) launch_rebuild(na);
+        const bool inline_rec = rec && !last && S <= kRecInline;
+        if (rec && !inline_rec) launch_record();
+        if (!last) launch_rebuild(na, inline_rec);
       }
     }
   }
@@ -1327,6 +1396,7 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   set_attr((const void*)k_b_fused<float, true, false, false>);
   set_attr((const void*)k_b_fused<float, false, true, false>);
   m.launch_fused(m.nbricks, 0.0, 0.0, 0, 1);
+  m.launch_record();
   LAUNCH_CHECK();
   HIP_TRY(hipStreamSynchronize(m.st));
   m.ready = true;
