@@ -72,7 +72,8 @@ def brick_lj(s, precision, steps, warmup, graph_steps=0):
     eng.fetch(state=False)
     out = {"steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
            "kernel_ms": ms / max(launches, 1), "launches": launches, "rebuilds": int(eng.rebuilds[0]),
-           "list_entries": eng.info["list_entries"], "energy_drift": eng.energy_drift()}
+           "list_entries": eng.info["list_entries"], "energy_drift": eng.energy_drift(),
+           "region_capacity": eng.info["region_capacity"], "rebuild_phase_us": eng.info["rebuild_phase_us"]}
     if graph_steps:
         blk = 10
         eng.run(blk, use_graph=True)   # capture + first replay

@@ -139,11 +139,16 @@ class BrickEngine:
         F = np.empty((N, 3)) if state else None
         log = np.zeros((S, self.log_cap, LOG_W))
         stats = np.zeros((S, 4), np.int64)
-        info = np.zeros(5, np.int64)
+        info = np.zeros(16, np.int64)
         L.call("accu_bmd_get", self._h, L.ptr(pos), L.ptr(vel), L.ptr(F), L.ptr(log), self.log_cap, L.ptr(stats),
                L.ptr(info))
+        khz = max(int(info[12]), 1)
+        ends = [1e3 * int(t) / khz for t in info[5:10]]
+        names = ("binning", "block_sums", "scan", "grouping", "order_and_gather")
         self.info = {"max_neighbours": int(info[0]), "capacity": int(info[1]), "region_capacity": int(info[2]),
-                     "list_entries": int(info[4])}
+                     "list_entries": int(info[4]),
+                     "rebuild_phase_us": {n: round(e - (ends[k - 1] if k else 0.0), 1)
+                                          for k, (n, e) in enumerate(zip(names, ends))}}
         self.rebuilds = stats[:, 0].copy()
         self.logs = [log[k, :int(stats[k, 1])].copy() for k in range(S)]
         if state:

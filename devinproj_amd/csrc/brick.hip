@@ -142,6 +142,16 @@ constexpr int kRB = 1024;  // threads per block of the rebuild launch: one block
                            // few blocks pay the cache write-back of each barrier
 constexpr int kBarSpin = 1 << 20;  // bounded barrier wait (0.1 s or more): a stuck barrier reports an error
 
+// Per-atom engine state except positions, double-buffered like the positions: system s
+// uses buffer spar[s]; a rebuild gathers into the other buffer and flips spar[s].
+struct BState {
+  double4* vel[2];
+  double4* F[2];
+  double4* ref[2];  // positions at the last list build
+  int* perm[2];     // user index of each slot
+  int* cell[2];     // cell id of each slot
+};
+
 struct RB {
   const BSys* sys;
   const BBrick* bricks;
@@ -152,26 +162,19 @@ struct RB {
   double4* posA;
   double4* posB;
   int* par;
-  double4* ref;
-  double4* vel;
-  double4* F;
-  int* perm;
-  int* cell;
-  double4* pos2;
-  double4* vel2;
-  double4* F2;
-  int* perm2;
-  int* cell2;
+  BState st;
+  int* spar;
   int* count;   // per cell; all zero between rebuilds
   int* start;   // [ncells + 1]
   int* cursor;  // [ncells + 1]
-  int* tmp;
-  int* sorted;
+  int* tmp;     // atoms grouped by cell
+  int* slot;    // each atom's arrival slot in its cell
   int* bsum;    // [grid]
   int* nn;
   uint4* nbr;   // groups of 8 u16 local indices: group g of atom a at [g * stride + a]
   int stride, ngroups, cap;
   int* ints;    // [0] max neighbours, [1] region overflow, [2] barrier timeout, [3] blocks seen then
+  long long* tphase;  // [0] start, [1..5] phase ends of the last rebuild (device wall clock)
   unsigned* bar;
   int* flags;   // [S] per system + [S] any, set by the fused step
   int* bflags;  // the same, as handed to the list launch
@@ -311,9 +314,10 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
                        -(1.0f + 0.5f * b.hd[2]) * h[2]};  // low corner of region cell 0
   const float rc2 = (float)(s.rc_list2 * (1.0 + 1e-5));
   const unsigned sentinel = (unsigned)total;
+  const int* __restrict__ cell = p.st.cell[p.spar[b.sys]];
   const int a0 = p.start[b.cell0], a1 = p.start[b.cell0 + b.nhome];
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-    const int local = p.cell[a] - b.cell0;
+    const int local = cell[a] - b.cell0;
     const int rh = home_region(b, local);
     const int self = rs[rh] + (a - p.start[b.cell0 + local]);
     const float4 xi = lds32[self];
@@ -334,17 +338,24 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
           if (ex * ex + ey * ey + ez * ez > rc2) continue;  // the cell's box is out of range
           const int r = (rx * b.rd[1] + ry) * b.rd[2] + rz;
           const int qe = rs[r + 1];
-          for (int q = rs[r]; q < qe; ++q) {
-            const float4 xj = lds32[q];
-            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            if (dx * dx + dy * dy + dz * dz < rc2 && q != self) {
-              w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-              w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-              w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-              w3 = __builtin_amdgcn_alignbit((unsigned)q, w3, 16);
-              if ((cnt & 7) == 7 && (cnt >> 3) < p.ngroups)
-                p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
-              ++cnt;
+          // four candidates per step: their LDS reads are in flight together (the slots
+          // past the cell's end re-read its last atom and are rejected by q < qe)
+          for (int q = rs[r]; q < qe; q += 4) {
+            float4 xj[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xj[k] = lds32[min(q + k, qe - 1)];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const float dx = xj[k].x - xi.x, dy = xj[k].y - xi.y, dz = xj[k].z - xi.z;
+              if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != self) {
+                w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+                w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+                w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+                w3 = __builtin_amdgcn_alignbit((unsigned)(q + k), w3, 16);
+                if ((cnt & 7) == 7 && (cnt >> 3) < p.ngroups)
+                  p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+                ++cnt;
+              }
             }
           }
         }
@@ -383,28 +394,51 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     return;
   }
   __shared__ int wsum[kRB / 64];
+  if (blockIdx.x == 0 && tid == 0) p.tphase[0] = wall_clock64();
   const int gtid = blockIdx.x * bd + tid, gsz = gridDim.x * bd;
-  // 1. cell of every atom of a flagged system; cells of the other systems keep their counts
-  for (int a = gtid; a < p.N; a += gsz) {
-    const int si = p.sysid[a];
-    if (!p.flags[si]) continue;
-    const BSys& s = p.sys[si];
-    const double4 x = (rb_par(p, si) ? p.posB : p.posA)[a];
-    const double xs[3] = {x.x, x.y, x.z};
-    int c[3];
+  // 1. cell of every atom of a flagged system and its arrival slot in that cell; cells of
+  //    the other systems keep their counts. Atoms are still nearly in cell order, so a
+  //    wave's lanes share a few cells: one atomic per distinct cell per wave (the lanes
+  //    of a cell take consecutive slots in lane order).
+  {
+    const int lane = tid & 63;
+    for (int a0 = gtid - lane; a0 < p.N; a0 += gsz) {
+      const int a = a0 + lane;
+      int gid = -1;
+      if (a < p.N) {
+        const int si = p.sysid[a];
+        if (p.flags[si]) {
+          const BSys& s = p.sys[si];
+          const double4 x = (rb_par(p, si) ? p.posB : p.posA)[a];
+          const double xs[3] = {x.x, x.y, x.z};
+          int c[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double t = xs[d] * s.cinv[d];
-      const int m = s.nc[d] - 1;
-      c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : (int)t);
+          for (int d = 0; d < 3; ++d) {
+            const double t = xs[d] * s.cinv[d];
+            const int m = s.nc[d] - 1;
+            c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : (int)t);
+          }
+          gid = p.cellmap[s.cellmap_off + (c[0] * s.nc[1] + c[1]) * s.nc[2] + c[2]];
+          p.st.cell[p.spar[si]][a] = gid;
+        }
+      }
+      unsigned long long todo = __ballot(gid >= 0);
+      while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int g = __shfl(gid, leader, 64);
+        const unsigned long long same = __ballot(gid == g);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&p.count[g], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if (gid == g) p.slot[a] = base + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+      }
     }
-    const int gid = p.cellmap[s.cellmap_off + (c[0] * s.nc[1] + c[1]) * s.nc[2] + c[2]];
-    p.cell[a] = gid;
-    atomicAdd(&p.count[gid], 1);
   }
   for (int c = gtid; c < p.ncells; c += gsz)
     if (!p.flags[p.cellsys[c]]) p.count[c] = p.start[c + 1] - p.start[c];
   if (!grid_sync(p.bar, p.ints + 2)) return;
+  if (blockIdx.x == 0 && tid == 0) p.tphase[1] = wall_clock64();
   // 2. exclusive scan of the counts over all cells (chunk per block), counts back to 0
   const int chunk = (p.ncells + gridDim.x - 1) / gridDim.x;
   const int c0 = min((int)blockIdx.x * chunk, p.ncells), c1 = min(c0 + chunk, p.ncells);
@@ -415,6 +449,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     if (tid == 0) p.bsum[blockIdx.x] = v;
   }
   if (!grid_sync(p.bar, p.ints + 2)) return;
+  if (blockIdx.x == 0 && tid == 0) p.tphase[2] = wall_clock64();
   {
     int off = 0;
     for (int k = tid; k < (int)blockIdx.x; k += bd) off += p.bsum[k];
@@ -426,76 +461,85 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       const int ex = block_excl_scan(v, wsum, tot);
       if (c < c1) {
         p.start[c] = off + ex;
-        p.cursor[c] = off + ex;
         p.count[c] = 0;
       }
       off += tot;
     }
-    if (c1 == p.ncells && tid == 0) {
-      p.start[p.ncells] = off;
-      p.cursor[p.ncells] = off;
-    }
+    if (c1 == p.ncells && tid == 0) p.start[p.ncells] = off;
   }
   if (p.bin_only) return;
   if (!grid_sync(p.bar, p.ints + 2)) return;
-  // 3. scatter atoms to their cells (order inside a cell fixed in 4)
-  for (int a = gtid; a < p.N; a += gsz)
-    if (p.flags[p.sysid[a]]) p.tmp[atomicAdd(&p.cursor[p.cell[a]], 1)] = a;
+  if (blockIdx.x == 0 && tid == 0) p.tphase[3] = wall_clock64();
+  // 3. atoms grouped by cell (arrival order inside a cell; fixed in 4)
+  for (int a = gtid; a < p.N; a += gsz) {
+    const int si = p.sysid[a];
+    if (p.flags[si]) p.tmp[p.start[p.st.cell[p.spar[si]][a]] + p.slot[a]] = a;
+  }
   if (!grid_sync(p.bar, p.ints + 2)) return;
-  // 4. stable order inside each cell: rank by original index, one wave per cell (the
-  //    cell's atoms in registers, compared through shuffles)
+  if (blockIdx.x == 0 && tid == 0) p.tphase[4] = wall_clock64();
+  // 4. rank each cell's atoms by original index and gather each atom's state to its
+  //    sorted slot in the other buffers; the list reference positions are the gathered
+  //    positions. When both cells of a pair have at most 32 atoms each half-wave takes one; otherwise
+  //    the whole wave takes the two in turn (shuffle rank up to 64 atoms, a rank against
+  //    the cell's atoms in memory beyond that). Control flow stays wave-uniform.
   {
-    const int lane = tid & 63;
+    const int lane = tid & 63, half = lane >> 5, hl = lane & 31;
     const int gw = gtid >> 6, nw = gsz >> 6;
-    for (int c = gw; c < p.ncells; c += nw) {
-      if (!p.flags[p.cellsys[c]]) continue;
-      const int a = p.start[c], len = p.start[c + 1] - a;
-      if (len <= 64) {
-        const int v = lane < len ? p.tmp[a + lane] : 0x7fffffff;
+    auto gather = [&](int c, int t, int v, int si) {
+      const int pc = rb_par(p, si), sc = p.spar[si];
+      const double4 x = (pc ? p.posB : p.posA)[v];
+      (pc ? p.posA : p.posB)[t] = x;
+      p.st.ref[sc ^ 1][t] = x;
+      p.st.vel[sc ^ 1][t] = p.st.vel[sc][v];
+      p.st.F[sc ^ 1][t] = p.st.F[sc][v];
+      p.st.perm[sc ^ 1][t] = p.st.perm[sc][v];
+      p.st.cell[sc ^ 1][t] = c;
+    };
+    for (int cp = 2 * gw; cp < p.ncells; cp += 2 * nw) {
+      const int cm = cp + half;  // this half-wave's cell
+      int len = 0, a = 0;
+      const bool on = cm < p.ncells && p.flags[p.cellsys[cm]];
+      if (on) {
+        a = p.start[cm];
+        len = p.start[cm + 1] - a;
+      }
+      if (!__any(len > 32)) {
+        const bool mine = on && hl < len;
+        const int v = mine ? p.tmp[a + hl] : 0x7fffffff;
         int r = 0;
-        for (int x = 0; x < len; ++x) r += (__shfl(v, x, 64) < v);
-        if (lane < len) p.sorted[a + r] = v;
+        for (int x = 0; x < 32; ++x) r += (__shfl(v, (half << 5) + x, 64) < v) ? 1 : 0;
+        if (mine) gather(cm, a + r, v, p.cellsys[cm]);
       } else {
-        for (int e = lane; e < len; e += 64) {
-          const int v = p.tmp[a + e];
-          int r = 0;
-          for (int x = 0; x < len; ++x) r += (p.tmp[a + x] < v);
-          p.sorted[a + r] = v;
+        for (int k = 0; k < 2; ++k) {
+          const int c = cp + k;
+          if (c >= p.ncells || !p.flags[p.cellsys[c]]) continue;
+          const int si = p.cellsys[c];
+          const int ca = p.start[c], cl = p.start[c + 1] - ca;
+          for (int e0 = 0; e0 < cl; e0 += 64) {
+            const int e = e0 + lane;
+            const bool mine = e < cl;
+            const int v = mine ? p.tmp[ca + e] : 0x7fffffff;
+            int r = 0;
+            if (cl <= 64) {
+              for (int x = 0; x < cl; ++x) r += (__shfl(v, x, 64) < v) ? 1 : 0;
+            } else if (mine) {
+              for (int x = 0; x < cl; ++x) r += (p.tmp[ca + x] < v) ? 1 : 0;
+            }
+            if (mine) gather(c, ca + r, v, si);
+          }
         }
       }
     }
   }
   if (!grid_sync(p.bar, p.ints + 2)) return;
-  // 5. gather the state into sorted order
-  for (int t = gtid; t < p.N; t += gsz) {
-    const int si = p.sysid[t];
-    if (!p.flags[si]) continue;
-    const int i = p.sorted[t];
-    p.pos2[t] = (rb_par(p, si) ? p.posB : p.posA)[i];
-    p.vel2[t] = p.vel[i];
-    p.F2[t] = p.F[i];
-    p.perm2[t] = p.perm[i];
-    p.cell2[t] = p.cell[i];
-  }
-  if (!grid_sync(p.bar, p.ints + 2)) return;
-  // 6. back into the live arrays; the list reference positions are the sorted ones
-  for (int t = gtid; t < p.N; t += gsz) {
-    const int si = p.sysid[t];
-    if (!p.flags[si]) continue;
-    (rb_par(p, si) ? p.posB : p.posA)[t] = p.pos2[t];
-    p.ref[t] = p.pos2[t];
-    p.vel[t] = p.vel2[t];
-    p.F[t] = p.F2[t];
-    p.perm[t] = p.perm2[t];
-    p.cell[t] = p.cell2[t];
-  }
-  if (!grid_sync(p.bar, p.ints + 2)) return;
-  // 7. bookkeeping: parity flips commit, flags move to the list launch, rebuilt systems
-  //    count one rebuild
+  if (blockIdx.x == 0 && tid == 0) p.tphase[5] = wall_clock64();
+  // 5. bookkeeping: parity flips commit (drift, and the gather for rebuilt systems),
+  //    flags move to the list launch, rebuilt systems count one rebuild
   if (blockIdx.x == 0) {
     for (int s = tid; s < p.S; s += bd) {
-      if (s < p.flip_n) p.par[s] ^= 1;
       const int f = p.flags[s];
+      p.par[s] ^= (s < p.flip_n ? 1 : 0) ^ (f ? 1 : 0);
+      if (f) p.spar[s] ^= 1;
       p.bflags[s] = f;
       if (f) {
         p.nreb[s] += 1;
@@ -615,9 +659,8 @@ template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
                                                  const BRegion* __restrict__ reg, const int* __restrict__ start,
                                                  double4* __restrict__ posA, double4* __restrict__ posB,
-                                                 const int* __restrict__ par, double4* __restrict__ vel,
-                                                 double4* __restrict__ Fst, const double4* __restrict__ ref,
-                                                 const int* __restrict__ cell, const int* __restrict__ nn,
+                                                 const int* __restrict__ par, BState sta,
+                                                 const int* __restrict__ spar, const int* __restrict__ nn,
                                                  const uint4* __restrict__ nbr, int stride, int cap, double wc,
                                                  double wn, int drift_from, int record, int* __restrict__ flags,
                                                  int S, double* __restrict__ partials, int* __restrict__ ints) {
@@ -632,6 +675,11 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   const int si = b.sys;
   const double4* __restrict__ pos = par[si] ? posB : posA;
   double4* __restrict__ pos_next = par[si] ? posA : posB;
+  const int sp = spar[si];
+  double4* __restrict__ vel = sta.vel[sp];
+  double4* __restrict__ Fst = sta.F[sp];
+  const double4* __restrict__ ref = sta.ref[sp];
+  const int* __restrict__ cell = sta.cell[sp];
   if constexpr (LJ_ON) {
     const int m = s.ns * s.ns;
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
@@ -762,8 +810,7 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
 // Opening half kick + drift of the first (sub)step for the atoms of the active systems.
 __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ sysid, int natoms,
                           double4* __restrict__ posA, double4* __restrict__ posB, const int* __restrict__ par,
-                          double4* __restrict__ vel, const double4* __restrict__ Fst, const double4* __restrict__ ref,
-                          double wn, int* __restrict__ flags, int S) {
+                          BState sta, const int* __restrict__ spar, double wn, int* __restrict__ flags, int S) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   bool moved_far = false;
   int si = 0;
@@ -772,9 +819,12 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
     const BSys& s = sys[si];
     const double4* pos = par[si] ? posB : posA;
     double4* pos_next = par[si] ? posA : posB;
+    const int sp = spar[si];
+    double4* vel = sta.vel[sp];
+    const double4* ref = sta.ref[sp];
     const double4 x0 = pos[a];
     const double m = s.mass[(int)x0.w];
-    const double4 f = Fst[a];
+    const double4 f = sta.F[sp][a];
     const double h = 0.5 * wn * s.dt, dtn = wn * s.dt;
     double4 v = vel[a], x = x0;
     {
@@ -803,14 +853,15 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
   }
 }
 
-// a (or, with b != NULL, the buffer par selects per system) scattered to user order
-__global__ void k_b_scatter_user(int natoms, const int* __restrict__ perm, const double4* __restrict__ a,
-                                 const double4* __restrict__ b, const int* __restrict__ par,
-                                 const int* __restrict__ sysid, double* __restrict__ out) {
+// Positions (which = 0), velocities (1) or forces (2) of every slot scattered to user order.
+__global__ void k_b_scatter_user(int natoms, int which, const double4* __restrict__ posA,
+                                 const double4* __restrict__ posB, const int* __restrict__ par, BState sta,
+                                 const int* __restrict__ spar, const int* __restrict__ sysid, double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= natoms) return;
-  const int u = perm[t];
-  const double4 v = (b && par[sysid[t]]) ? b[t] : a[t];
+  const int si = sysid[t], sp = spar[si];
+  const int u = sta.perm[sp][t];
+  const double4 v = which == 0 ? (par[si] ? posB : posA)[t] : (which == 1 ? sta.vel[sp] : sta.F[sp])[t];
   out[3 * (size_t)u] = v.x;
   out[3 * (size_t)u + 1] = v.y;
   out[3 * (size_t)u + 2] = v.z;
@@ -845,11 +896,11 @@ struct BMD {
   int ncells = 0, nbricks = 0;
   std::vector<int> brick_off, atom_off;
   // state
-  double4 *pos = nullptr, *pos_b = nullptr, *vel = nullptr, *F = nullptr, *ref = nullptr;  // pos/pos_b: buffers A/B
-  int* par = nullptr;  // per system: 1 when its current positions are in pos_b
-  double4 *pos2 = nullptr, *vel2 = nullptr, *F2 = nullptr;
-  int *perm = nullptr, *perm2 = nullptr, *cell = nullptr, *cell2 = nullptr;
-  int *count = nullptr, *start = nullptr, *cursor = nullptr, *tmp = nullptr, *sorted = nullptr;
+  double4 *pos = nullptr, *pos_b = nullptr;  // position buffers A/B
+  int* par = nullptr;                         // per system: 1 when its current positions are in pos_b
+  BState sta = {};                            // the rest of the per-atom state, buffers [0] / [1]
+  int* spar = nullptr;                        // per system: which state buffer is current
+  int *count = nullptr, *start = nullptr, *tmp = nullptr, *slot = nullptr;
   int* nn = nullptr;
   uint4* nbr = nullptr;  // groups of 8 u16 local indices, group g of atom a at [g * stride + a]
   int stride = 0, max_nbrs = 0;  // max_nbrs: list capacity per atom, a multiple of 8
@@ -859,6 +910,7 @@ struct BMD {
   int* bflags = nullptr;   // rebuild flags handed to the list launch
   unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
   int* bsum = nullptr;      // per-block sums of the rebuild scan
+  long long* tphase = nullptr;  // rebuild phase timestamps
   static constexpr int kMaxGrid = 1 << 16;
   int grid_rb = 0;  // resident grid of the rebuild launch
   int* log_count = nullptr;
@@ -876,9 +928,10 @@ struct BMD {
   std::vector<long long> graph_key;
 
   void release() {
-    void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, vel, F, ref, pos2, vel2, F2,
-                  perm, perm2, cell, cell2, count, start, cursor, tmp, sorted, nn, nbr, flags, ints, nreb, log_count,
-                  log, partials, bflags, bar, bsum};
+    void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, spar, sta.vel[0],
+                  sta.vel[1], sta.F[0], sta.F[1], sta.ref[0], sta.ref[1], sta.perm[0], sta.perm[1], sta.cell[0],
+                  sta.cell[1], count, start, tmp, slot, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
+                  bar, bsum, tphase};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -901,21 +954,12 @@ struct BMD {
     p.posA = pos;
     p.posB = pos_b;
     p.par = par;
-    p.ref = ref;
-    p.vel = vel;
-    p.F = F;
-    p.perm = perm;
-    p.cell = cell;
-    p.pos2 = pos2;
-    p.vel2 = vel2;
-    p.F2 = F2;
-    p.perm2 = perm2;
-    p.cell2 = cell2;
+    p.st = sta;
+    p.spar = spar;
     p.count = count;
     p.start = start;
-    p.cursor = cursor;
     p.tmp = tmp;
-    p.sorted = sorted;
+    p.slot = slot;
     p.bsum = bsum;
     p.nn = nn;
     p.nbr = nbr;
@@ -923,6 +967,7 @@ struct BMD {
     p.ngroups = max_nbrs / 8;
     p.cap = cap_build;
     p.ints = ints;
+    p.tphase = tphase;
     p.bar = bar;
     p.flags = flags;
     p.bflags = bflags;
@@ -968,15 +1013,15 @@ struct BMD {
 
     if (lj && co)
       hipLaunchKernelGGL((k_b_fused<P, true, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
-                         pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
+                         pos, pos_b, par, sta, spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
                          flags, S, partials, ints);
     else if (lj)
       hipLaunchKernelGGL((k_b_fused<P, true, false, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
-                         pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
+                         pos, pos_b, par, sta, spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
                          flags, S, partials, ints);
     else
       hipLaunchKernelGGL((k_b_fused<P, false, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
-                         pos, pos_b, par, vel, F, ref, cell, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
+                         pos, pos_b, par, sta, spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
                          flags, S, partials, ints);
   }
 
@@ -1018,8 +1063,8 @@ struct BMD {
   // buffer parity flips with every drift, on the device (inside the rebuild launch).
   void enqueue(int coarse, int nticks, const int* active, const double* wc, const double* wn, int record_every) {
     const int T = 256;
-    hipLaunchKernelGGL(k_b_drift, dim3((N + T - 1) / T), dim3(T), 0, st, d_sys, d_sysid, N, pos, pos_b, par, vel, F,
-                       ref, wc[0], flags, S);
+    hipLaunchKernelGGL(k_b_drift, dim3((N + T - 1) / T), dim3(T), 0, st, d_sys, d_sysid, N, pos, pos_b, par, sta, spar,
+                       wc[0], flags, S);
     launch_rebuild(S);
     for (int c = 0; c < coarse; ++c) {
       for (int t = 0; t < nticks; ++t) {
@@ -1193,14 +1238,16 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
   ARG_CHECK(N < (1LL << 30), "too many atoms");
   m->N = (int)N;
   const size_t d4 = (size_t)N * sizeof(double4), i4 = (size_t)N * sizeof(int);
-  void** d4s[] = {(void**)&m->pos, (void**)&m->pos_b, (void**)&m->vel, (void**)&m->F, (void**)&m->ref,
-                  (void**)&m->pos2, (void**)&m->vel2, (void**)&m->F2};
+  void** d4s[] = {(void**)&m->pos,        (void**)&m->pos_b,      (void**)&m->sta.vel[0], (void**)&m->sta.vel[1],
+                  (void**)&m->sta.F[0],   (void**)&m->sta.F[1],   (void**)&m->sta.ref[0], (void**)&m->sta.ref[1]};
   for (void** p : d4s) HIP_TRY(hipMalloc(p, d4));
-  void** i4s[] = {(void**)&m->perm, (void**)&m->perm2, (void**)&m->cell, (void**)&m->cell2, (void**)&m->tmp,
-                  (void**)&m->sorted, (void**)&m->nn};
+  void** i4s[] = {(void**)&m->sta.perm[0], (void**)&m->sta.perm[1], (void**)&m->sta.cell[0], (void**)&m->sta.cell[1],
+                  (void**)&m->tmp,         (void**)&m->slot,        (void**)&m->nn};
   for (void** p : i4s) HIP_TRY(hipMalloc(p, i4 + 64));
   HIP_TRY(hipMalloc(&m->par, nsys * sizeof(int) + 64));
   HIP_TRY(hipMemset(m->par, 0, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMalloc(&m->spar, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMemset(m->spar, 0, nsys * sizeof(int) + 64));
   HIP_TRY(hipMalloc(&m->flags, (nsys + 1) * sizeof(int) + 64));
   HIP_TRY(hipMemset(m->flags, 0, (nsys + 1) * sizeof(int) + 64));
   HIP_TRY(hipMalloc(&m->ints, 64));
@@ -1217,9 +1264,9 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
   }
   HIP_TRY(hipMemcpy(m->pos, h4.data(), d4, hipMemcpyHostToDevice));
   for (long long i = 0; i < N; ++i) h4[i] = make_double4(vel_host[3 * i], vel_host[3 * i + 1], vel_host[3 * i + 2], 0);
-  HIP_TRY(hipMemcpy(m->vel, h4.data(), d4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(m->F, 0, d4));
-  HIP_TRY(hipMemcpy(m->perm, perm.data(), i4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m->sta.vel[0], h4.data(), d4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(m->sta.F[0], 0, d4));
+  HIP_TRY(hipMemcpy(m->sta.perm[0], perm.data(), i4, hipMemcpyHostToDevice));
   HIP_TRY(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
   // species indices are checked when the species table arrives
   m->spec[0].ns = 0;
@@ -1306,7 +1353,6 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   if (rc) return rc;
   HIP_TRY(hipMalloc(&m.count, (m.ncells + 1) * sizeof(int)));
   HIP_TRY(hipMalloc(&m.start, (m.ncells + 1) * sizeof(int)));
-  HIP_TRY(hipMalloc(&m.cursor, (m.ncells + 1) * sizeof(int)));
   HIP_TRY(hipMalloc(&m.partials, (size_t)m.nbricks * 5 * sizeof(double)));
   m.log_cap = std::max(log_cap, 4);
   HIP_TRY(hipMalloc(&m.log, (size_t)m.S * m.log_cap * kRecW * sizeof(double)));
@@ -1317,6 +1363,8 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   HIP_TRY(hipMalloc(&m.bar, 64));
   HIP_TRY(hipMemset(m.bar, 0, 64));
   HIP_TRY(hipMalloc(&m.bsum, BMD::kMaxGrid * sizeof(int)));
+  HIP_TRY(hipMalloc(&m.tphase, 16 * sizeof(long long)));
+  HIP_TRY(hipMemset(m.tphase, 0, 16 * sizeof(long long)));
   // first binning of every system: measure the region sizes to size the LDS
   m.grid_rb = m.resident_grid();
   ARG_CHECK(m.grid_rb > 0, "the rebuild launch (%d threads per block) does not fit on a CU", kRB);
@@ -1466,13 +1514,11 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
   const int N = m.N;
   double* tmp = nullptr;
   HIP_TRY(hipMallocAsync((void**)&tmp, (size_t)N * 3 * sizeof(double), m.st));
-  double4* srcs[3] = {m.pos, m.vel, m.F};
-  double4* alts[3] = {m.pos_b, nullptr, nullptr};
   double* dsts[3] = {pos_host, vel_host, F_host};
   for (int k = 0; k < 3; ++k) {
     if (!dsts[k]) continue;
-    hipLaunchKernelGGL(k_b_scatter_user, dim3((N + 255) / 256), dim3(256), 0, m.st, N, m.perm, srcs[k], alts[k], m.par,
-                       m.d_sysid, tmp);
+    hipLaunchKernelGGL(k_b_scatter_user, dim3((N + 255) / 256), dim3(256), 0, m.st, N, k, m.pos, m.pos_b, m.par,
+                       m.sta, m.spar, m.d_sysid, tmp);
     HIP_TRY(hipMemcpyAsync(dsts[k], tmp, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, m.st));
   }
   std::vector<int> nreb(m.S), lc(m.S);
@@ -1508,6 +1554,15 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
     long long tot = 0;
     for (int v : nn) tot += v;
     info[4] = tot;
+    // [5..9] phase ends of the last rebuild, [12] wall-clock rate (kHz)
+    long long tp[6];
+    HIP_TRY(hipMemcpy(tp, m.tphase, sizeof tp, hipMemcpyDeviceToHost));
+    for (int k = 1; k < 6; ++k) info[4 + k] = tp[k] - tp[0];
+    info[10] = info[11] = 0;
+    int dev = 0, khz = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    info[12] = khz;
   }
   if (ints[2] != 0) {
     set_error("rebuild grid barrier timed out: %d of %d blocks arrived", ints[3], m.grid_rb);

@@ -142,7 +142,10 @@ int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active_host, co
                  const double* wn_host, int record_every, int use_graph);
 int accu_bmd_sync(accu_bmd* h);
 /* state in user order; log_host [nsys][log_max][8]; stats [nsys][4] (rebuilds, records);
- * info [5] = max neighbours, capacity, region capacity, region overflow, list entries */
+ * info [16] = max neighbours, capacity, region capacity, region overflow, list entries,
+ * then the last rebuild's phase ends in device wall-clock ticks after its start
+ * ([5] binning, [6] block sums, [7] scan, [8] grouping by cell, [9] in-cell order and
+ * gather), [12] the wall-clock rate in kHz */
 int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
                  long long* stats_host, long long* info_host);
 /* workgroup size of the per-brick kernels (multiple of 64 in [64, 512]; default 256) */
