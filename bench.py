@@ -40,13 +40,19 @@ HBM_PEAK_GBS = 8000.0
 # summary with its commands is profiles/k_b_fused_lj_argon_1048576.md
 FUSED_TRAFFIC_BYTES = 5.793e8
 DT = 10e-15
-# tile choice of the autotuner on MI355X for this workload (devinproj_amd/autotune.py)
-LJ_BRICKS = (2, 2, 3)
-LJ_THREADS = 256
+# brick tiles: the autotuner's choice on MI355X for each workload and precision
+# (python -m devinproj_amd.autotune writes devinproj_amd/data/tiles_mi355x.json); the
+# fallback is the best tile of a manual sweep on the 1M-atom argon workload
+FALLBACK_TILE = {"bricks": (3, 3, 3), "threads": 512}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def tile_for(system, forces, precision):
+    from devinproj_amd import autotune
+    return autotune.cached_tile(system, forces, precision, autotune.DEFAULT_CACHE) or FALLBACK_TILE
 
 
 def brick_lj(s, precision, steps, warmup, graph_steps=0):
@@ -56,8 +62,9 @@ def brick_lj(s, precision, steps, warmup, graph_steps=0):
     from devinproj_amd import batch
     from devinproj_amd import workloads as W
     lj = dp.LennardJones(W.argon_params(), cutoff=2.5, skin=0.3, precision=precision)
-    eng = batch.BrickEngine([s], [[lj]], dp.VelocityVerlet(), DT, precision=precision, bricks=[LJ_BRICKS],
-                            threads=LJ_THREADS, log_cap=steps + warmup + graph_steps + 64)
+    tile = tile_for(s, [lj], precision)
+    eng = batch.BrickEngine([s], [[lj]], dp.VelocityVerlet(), DT, precision=precision, bricks=[tile["bricks"]],
+                            threads=tile["threads"], log_cap=steps + warmup + graph_steps + 64)
     eng.run(warmup)
     eng.set_timing(True)
     torch.cuda.synchronize()
@@ -70,7 +77,8 @@ def brick_lj(s, precision, steps, warmup, graph_steps=0):
     ms, launches = eng.timing()
     eng.set_timing(False)
     eng.fetch(state=False)
-    out = {"steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
+    out = {"tile": {"bricks": list(tile["bricks"]), "threads": tile["threads"]},
+           "steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
            "kernel_ms": ms / max(launches, 1), "launches": launches, "rebuilds": int(eng.rebuilds[0]),
            "list_entries": eng.info["list_entries"], "energy_drift": eng.energy_drift(),
            "region_capacity": eng.info["region_capacity"], "rebuild_phase_us": eng.info["rebuild_phase_us"]}
@@ -94,15 +102,19 @@ def brick_nacl(precision, steps, warmup):
     s = W.rocksalt(32, 32, 64, seed=3)
     forces = [dp.LennardJones(W.NACL_LJ, cutoff=2.5, precision=precision),
               dp.DampedShiftedCoulomb(rcut=12e-10, alpha=0.2e10, precision=precision)]
-    eng = batch.BrickEngine([s], [forces], dp.VelocityVerlet(), 1e-15, precision=precision, bricks=[(1, 1, 1)],
-                            threads=256, log_cap=steps + warmup + 16)
+    tile = tile_for(s, forces, precision)
+    if tile is FALLBACK_TILE:
+        tile = {"bricks": (1, 1, 1), "threads": 256}
+    eng = batch.BrickEngine([s], [forces], dp.VelocityVerlet(), 1e-15, precision=precision, bricks=[tile["bricks"]],
+                            threads=tile["threads"], log_cap=steps + warmup + 16)
     eng.run(warmup)
     t0 = time.perf_counter()
     eng.run(steps, sync=False)
     eng.sync()
     t1 = time.perf_counter()
     eng.fetch(state=False)
-    out = {"steps_per_s": steps / (t1 - t0), "rebuilds": int(eng.rebuilds[0]), "energy_drift": eng.energy_drift()}
+    out = {"steps_per_s": steps / (t1 - t0), "rebuilds": int(eng.rebuilds[0]), "energy_drift": eng.energy_drift(),
+           "tile": {"bricks": list(tile["bricks"]), "threads": tile["threads"]}}
     eng.close()
     return out
 
@@ -236,7 +248,7 @@ def main():
         "data": "synthetic (seeded FCC argon, jitter 0.01 sigma, 94.4 K)",
         "config": {"workload": "lj-argon-fcc-1048576-vv-dt10fs", "atoms": n, "cutoff_sigma": 2.5,
                    "skin_sigma": 0.3, "precision": "mixed", "record_every": 1, "engine": "brick",
-                   "bricks": list(LJ_BRICKS), "threads": LJ_THREADS, "device": name},
+                   "bricks": mixed["tile"]["bricks"], "threads": mixed["tile"]["threads"], "device": name},
         "pair_interactions_per_s": npairs * mixed["steps_per_s"],
         "fp64_steps_per_s": fp64["steps_per_s"] if fp64 else None,
         "mixed_over_fp64": mixed["steps_per_s"] / fp64["steps_per_s"] if fp64 else None,

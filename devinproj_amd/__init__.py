@@ -5,11 +5,12 @@ a device-resident fused MD engine, and the accuracy-report rows on this path.
 The HIP library (_accuhip.so) is required; there is no CPU fallback.
 """
 from .accuracy import AccuracyReport, Check, timestep_convergence
+from .batch import BrickEngine
 from .engine import MDEngine
 from .forces import (DampedShiftedCoulomb, Ewald, Force, Gravity, LennardJones, NeighbourList, VerletList,
                      cell_list_pairs, kvector_columns)
 from .integrators import Integrator, VelocityVerlet, Yoshida4
-from . import workloads
+from . import autotune, batch, workloads
 from .learned import LearnedDSF, PairCorrection, train_correction
 from .simulation import Simulation, Trajectory
 from .system import Species, System
@@ -17,7 +18,7 @@ from .tree import BarnesHut, direct_subset
 from .units import G, k_B, k_e, si
 
 __all__ = [
-    "AccuracyReport", "BarnesHut", "Check", "DampedShiftedCoulomb", "Ewald", "Force", "G", "Gravity", "Integrator",
+    "AccuracyReport", "BarnesHut", "BrickEngine", "Check", "DampedShiftedCoulomb", "Ewald", "Force", "G", "Gravity", "Integrator",
     "LearnedDSF", "LennardJones", "MDEngine", "NeighbourList", "PairCorrection", "Simulation", "Species", "System",
     "Trajectory", "VelocityVerlet", "train_correction",
     "VerletList", "Yoshida4", "cell_list_pairs", "direct_subset", "k_B", "k_e", "kvector_columns", "si",

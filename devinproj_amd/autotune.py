@@ -1,23 +1,32 @@
-"""Autotuner for the brick engine's tile: the brick shape (cells per axis) of each system.
+"""Autotuner for the brick engine's tile: the brick shape (cells per axis) and the
+workgroup size of the per-brick kernels.
 
-`choose_bricks(system, forces, precision)` builds a BrickEngine for every candidate
-shape whose staged region fits the LDS, times a few fused steps with HIP events, and
-returns the fastest. Results are cached per size bucket (atoms per cell, log2 of N,
-precision, interactions) in memory and optionally in a JSON file, so a batch of many
-systems tunes one representative per bucket (`tune_batch`).
+`choose_tile(system, make_forces, precision, dt)` builds a BrickEngine for every
+candidate (each brick shape whose staged region fits the LDS, at 256 and 512 threads),
+times whole MD steps on the wall clock (fused steps, records and the rebuilds that
+happen in the run), and returns the fastest as {"bricks": (bx, by, bz), "threads": t}.
+Results are cached per size bucket (atoms per cell, log2 N, precision, interactions) in
+memory and optionally in a JSON file, so a batch of many systems tunes one
+representative per bucket (`tune_batch`). The list layout (16-byte groups of eight
+16-bit entries, one group row per `stride` atoms) is fixed, so the list stride is not
+a tuning parameter.
 """
 from __future__ import annotations
 
 import json
 import math
 import os
+import time
 
 import numpy as np
 
 from .batch import BrickEngine
 from .integrators import VelocityVerlet
 
-CANDIDATES = [(1, 1, 1), (1, 1, 2), (1, 2, 2), (2, 2, 2), (2, 2, 3), (2, 3, 3), (3, 3, 3)]
+CANDIDATES = [(1, 1, 1), (1, 1, 2), (1, 2, 2), (2, 2, 2), (2, 2, 3), (2, 3, 3), (3, 3, 3), (3, 3, 4), (3, 4, 4),
+              (4, 4, 4)]
+THREADS = (256, 512)
+LDS_BYTES = 160 * 1024
 _CACHE: dict = {}
 
 
@@ -42,7 +51,9 @@ def bucket_key(system, forces, precision):
     return f"{precision}|{kinds}|cell{int(round(per_cell))}|n2^{int(math.log2(max(system.n, 2)))}"
 
 
-def candidates_for(system, forces, precision, lds_bytes=160 * 1024):
+def candidates_for(system, forces, precision, lds_bytes=LDS_BYTES, slack=0.25):
+    """Brick shapes that fit the cell grid and whose staged region (with the engine's
+    slack and room for density fluctuations) fits the LDS in the fused and list kernels."""
     rc = _list_radius(system, forces)
     nc = np.floor(np.asarray(system.box, float) / rc).astype(int)
     per_cell = system.n / float(np.prod(nc))
@@ -51,60 +62,131 @@ def candidates_for(system, forces, precision, lds_bytes=160 * 1024):
     for b in CANDIDATES:
         if any(b[d] > nc[d] for d in range(3)):
             continue
-        region = np.prod(np.asarray(b) + 2) * per_cell * 1.5 + 64
-        if region * 32 + 4096 <= lds_bytes and region * elem + 4096 <= lds_bytes:
+        region = np.prod(np.asarray(b) + 2) * per_cell * (1.0 + slack) * 1.1 + 64
+        if region * max(elem, 16) + 4096 <= lds_bytes:
             out.append(b)
     return out or [(1, 1, 1)]
 
 
-def time_bricks(system, make_forces, precision, bricks, dt, steps=20):
+def time_tile(system, make_forces, precision, bricks, threads, dt, steps=40):
+    """Wall-clock seconds per MD step (VelocityVerlet) of one engine with this tile."""
     s = system.copy()
-    eng = BrickEngine([s], [make_forces(s)], VelocityVerlet(), dt, precision=precision, bricks=[bricks])
+    eng = BrickEngine([s], [make_forces(s)], VelocityVerlet(), dt, precision=precision, bricks=[bricks],
+                      threads=threads, log_cap=steps + 16)
     try:
-        eng.run(3)
-        eng.set_timing(True)
+        eng.run(5)
+        t0 = time.perf_counter()
         eng.run(steps)
-        ms, k = eng.timing()
-        return ms / max(k, 1)
+        t1 = time.perf_counter()
+        eng.fetch(state=False)  # raises if a region or list overflowed
+        return (t1 - t0) / steps
     finally:
         eng.close()
 
 
-def choose_bricks(system, make_forces, precision, dt, cache_file=None, steps=20, log=None):
-    key = bucket_key(system, make_forces(system), precision)
+def _load(cache_file):
     if cache_file and not _CACHE and os.path.exists(cache_file):
         with open(cache_file) as fh:
-            _CACHE.update({k: tuple(v) for k, v in json.load(fh).items()})
+            for k, v in json.load(fh).items():
+                _CACHE[k] = {"bricks": tuple(v["bricks"]), "threads": int(v["threads"])}
+
+
+def _save(cache_file):
+    if cache_file:
+        with open(cache_file, "w") as fh:
+            json.dump({k: {"bricks": list(v["bricks"]), "threads": v["threads"]} for k, v in _CACHE.items()}, fh,
+                      indent=1)
+
+
+def cached_tile(system, forces, precision, cache_file):
+    """The cached tile for this system's bucket, or None."""
+    _load(cache_file)
+    return _CACHE.get(bucket_key(system, forces, precision))
+
+
+def choose_tile(system, make_forces, precision, dt, cache_file=None, steps=40, threads=THREADS, log=None):
+    key = bucket_key(system, make_forces(system), precision)
+    _load(cache_file)
     if key in _CACHE:
         return _CACHE[key]
     best, best_t = None, float("inf")
     for b in candidates_for(system, make_forces(system), precision):
-        try:
-            t = time_bricks(system, make_forces, precision, b, dt, steps)
-        except (RuntimeError, ValueError) as e:  # region does not fit this shape
+        for t in threads:
+            try:
+                sec = time_tile(system, make_forces, precision, b, t, dt, steps)
+            except (RuntimeError, ValueError) as e:  # region or list does not fit this tile
+                if log:
+                    log(f"  bricks {b} threads {t}: skipped ({e})")
+                continue
             if log:
-                log(f"  bricks {b}: skipped ({e})")
-            continue
-        if log:
-            log(f"  bricks {b}: {t:.4f} ms per fused step")
-        if t < best_t:
-            best, best_t = b, t
+                log(f"  bricks {b} threads {t}: {1e3 * sec:.4f} ms per step")
+            if sec < best_t:
+                best, best_t = {"bricks": b, "threads": t}, sec
+    if best is None:
+        raise RuntimeError(f"no brick tile fits system bucket {key}")
     _CACHE[key] = best
-    if cache_file:
-        with open(cache_file, "w") as fh:
-            json.dump({k: list(v) for k, v in _CACHE.items()}, fh, indent=1)
+    _save(cache_file)
     return best
 
 
+def choose_bricks(system, make_forces, precision, dt, cache_file=None, steps=40, log=None):
+    """Brick shape of the fastest tile (see choose_tile)."""
+    return choose_tile(system, make_forces, precision, dt, cache_file=cache_file, steps=steps, log=log)["bricks"]
+
+
 def tune_batch(systems, make_forces, precision, dt, cache_file=None, log=None):
-    """One brick shape per system, tuning one representative (the largest) per bucket."""
+    """One brick shape per system, tuning one representative (the largest) per bucket;
+    returns (bricks per system, threads), the threads being those of the bucket with the
+    most atoms in total (one workgroup size serves the whole batch)."""
     keys = [bucket_key(s, make_forces(s), precision) for s in systems]
-    rep = {}
+    rep, atoms = {}, {}
     for s, k in zip(systems, keys):
         if k not in rep or s.n > rep[k].n:
             rep[k] = s
+        atoms[k] = atoms.get(k, 0) + s.n
     for k, s in rep.items():
         if log:
             log(f"tuning bucket {k} on a {s.n}-atom system")
-        choose_bricks(s, make_forces, precision, dt, cache_file=cache_file, log=log)
-    return [_CACHE[k] for k in keys]
+        choose_tile(s, make_forces, precision, dt, cache_file=cache_file, log=log)
+    heavy = max(atoms, key=atoms.get)
+    return [_CACHE[k]["bricks"] for k in keys], _CACHE[heavy]["threads"]
+
+
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tiles_mi355x.json")
+
+
+def _bench_workloads():
+    """The LJ argon and molten NaCl systems that bench.py measures, with their forces."""
+    from . import forces as F
+    from . import workloads as W
+    argon = W.argon(64, seed=2, jitter=0.01)
+    nacl = W.rocksalt(32, 32, 64, seed=3)
+
+    def lj_forces(prec):
+        return lambda s: [F.LennardJones(W.argon_params(), cutoff=2.5, skin=0.3, precision=prec)]
+
+    def nacl_forces(prec):
+        return lambda s: [F.LennardJones(W.NACL_LJ, cutoff=2.5, precision=prec),
+                          F.DampedShiftedCoulomb(rcut=12e-10, alpha=0.2e10, precision=prec)]
+
+    return [(argon, lj_forces, 10e-15), (nacl, nacl_forces, 1e-15)]
+
+
+def main(argv=None):
+    """python -m devinproj_amd.autotune [--cache FILE]: tune the benchmark workloads."""
+    import argparse
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cache", default=DEFAULT_CACHE)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    for system, make, dt in _bench_workloads():
+        for prec in ("mixed", "fp64"):
+            log(f"{system.n} atoms, {prec}")
+            tile = choose_tile(system, make(prec), prec, dt, cache_file=a.cache, log=log)
+            print(json.dumps({"atoms": system.n, "precision": prec, "bricks": list(tile["bricks"]),
+                              "threads": tile["threads"]}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

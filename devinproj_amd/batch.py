@@ -9,7 +9,7 @@ independent systems in one device arena (the C ABI `accu_bmd_*`). Typical uses:
     one schedule (4 ticks per coarse step: dt/4 runs every tick, dt/2 every other
     tick, dt once), optionally as replays of one captured hipGraph, and evaluates
     the reference's Richardson formula on the final positions.
-Brick shapes (the LDS tile) are per system; `autotune.choose_bricks` picks them by
+Brick shapes (the LDS tile) are per system; `autotune.choose_tile` picks them (and the workgroup size) by
 timing candidates.
 """
 from __future__ import annotations
