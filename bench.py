@@ -1,7 +1,7 @@
 """Benchmark: MD steps/s of the device-resident fused LJ path on 1,048,576 argon atoms.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-dsf] [--no-fp64] [--no-learned]
-                    [--no-graph] [--copy-probe MIB]
+                    [--no-graph] [--no-drift] [--no-richardson] [--no-gravity] [--copy-probe MIB]
 
 Workload (one step = one velocity-Verlet step of the whole system, state resident in
 HBM, energies recorded every step): 64^3 x 4 = 1,048,576-atom FCC argon at
@@ -119,6 +119,101 @@ def brick_nacl(precision, steps, warmup):
     return out
 
 
+def drift_long(s, precision, steps=10_000):
+    """Energy drift (accuracy.py:72-76 formula) over `steps` recorded VV steps of the 1M-atom
+    workload, state resident in HBM."""
+    import devinproj_amd as dp
+    from devinproj_amd import batch
+    from devinproj_amd import workloads as W
+    lj = dp.LennardJones(W.argon_params(), cutoff=2.5, skin=0.3, precision=precision)
+    tile = tile_for(s, [lj], precision)
+    eng = batch.BrickEngine([s], [[lj]], dp.VelocityVerlet(), DT, precision=precision, bricks=[tile["bricks"]],
+                            threads=tile["threads"], log_cap=steps + 64)
+    t0 = time.perf_counter()
+    eng.run(steps, record_every=1)
+    t1 = time.perf_counter()
+    eng.fetch(state=False)
+    out = {"steps": steps, "energy_drift": eng.energy_drift(), "rebuilds": int(eng.rebuilds[0]),
+           "steps_per_s": steps / (t1 - t0)}
+    eng.close()
+    return out
+
+
+def richardson_batch(coarse_steps=2000, dt=0.005, precision="mixed"):
+    """48 uneven reduced-unit LJ systems (N log-uniform in [4000, 262144], rho* in
+    [0.7, 0.9], T* = 1), each at dt, dt/2, dt/4 (144 trajectories) in one batched engine,
+    `coarse_steps` steps at the coarsest dt, blocks of 10 captured as one hipGraph;
+    timestep_convergence (accuracy.py:191-221) per system."""
+    import devinproj_amd as dp
+    from devinproj_amd import autotune, batch
+    from devinproj_amd import workloads as W
+    systems = [W.lj_reduced(n, rho, T=1.0, seed=sd) for n, rho, sd in W.uneven_batch()]
+    make = lambda sy: [dp.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3, precision=precision)]
+    tiles = [autotune.cached_tile(sy, make(sy), precision, autotune.DEFAULT_CACHE) for sy in systems]
+    bricks = [t["bricks"] if t else None for t in tiles]  # None: the engine's default shape
+    t0 = time.perf_counter()
+    checks, eng, _ = batch.richardson(systems, make, dt, coarse_steps, precision=precision, bricks=bricks,
+                                      use_graph=True, block=10, threads=256)
+    t1 = time.perf_counter()
+    n_atoms = sum(sy.n for sy in systems)
+    atom_steps = n_atoms * coarse_steps * (1 + 2 + 4)
+    status = {}
+    for c in checks:
+        status[c.status] = status.get(c.status, 0) + 1
+    est = sorted(c.value for c in checks if c.value is not None)
+    out = {"systems": len(systems), "trajectories": 3 * len(systems), "atoms_per_replica_set": n_atoms,
+           "coarse_steps": coarse_steps, "run_seconds": eng.run_seconds, "total_seconds": t1 - t0,
+           "coarse_steps_per_s": coarse_steps / eng.run_seconds, "atom_steps_per_s": atom_steps / eng.run_seconds,
+           "status_counts": status, "median_estimate": est[len(est) // 2] if est else None}
+    eng.close()
+    return out
+
+
+def gravity_plummer(n=262144, seed=11):
+    """Plummer sphere of n equal-mass stars (1e5 Msun, a = 1 pc), softening 0.01 pc: direct
+    O(N^2) forces (float64 and mixed), Barnes-Hut theta = 0.5 with quadrupoles and its RMS
+    force error against direct sum on 200 sampled stars (accuracy.py:135-146), and one
+    Yoshida4 step with direct forces."""
+    import torch
+
+    import devinproj_amd as dp
+    from devinproj_amd import workloads as W
+    from devinproj_amd.accuracy import rms_rel_error
+    from devinproj_amd.units import Msun, parsec
+    s, a, M = W.plummer(n, a=parsec, m=1e5 * Msun / n, seed=seed)
+    soft = 0.01 * parsec
+
+    def timed(fn, reps=2):
+        fn(s)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn(s)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    out = {"stars": n}
+    for prec in ("mixed", "fp64"):
+        g = dp.Gravity(softening=soft, precision=prec)
+        sec, _ = timed(g.compute)
+        out[f"direct_{prec}_ms"] = 1e3 * sec
+        out[f"direct_{prec}_interactions_per_s"] = n * (n - 1) / sec
+    bh = dp.BarnesHut(softening=soft, theta=0.5, quadrupole=True)
+    sec, (Fb, _) = timed(bh.compute)
+    out["bh_theta05_quad_ms"] = 1e3 * sec
+    rng = np.random.default_rng(0)
+    sub = rng.choice(n, 200, replace=False)
+    Fd = dp.direct_subset(dp.Gravity(softening=soft), s, sub)
+    out["bh_rms_rel_force_error"] = rms_rel_error(Fb[sub], Fd)
+    t_cross = float(np.sqrt(a ** 3 / (dp.G * M)))
+    sim = dp.Simulation(s, [dp.Gravity(softening=soft, precision="mixed")], integrator=dp.Yoshida4(),
+                        dt=1e-3 * t_cross)
+    t0 = time.perf_counter()
+    sim.run(steps=2, positions=False)
+    out["yoshida4_direct_mixed_s_per_step"] = (time.perf_counter() - t0) / 2
+    return out
+
+
 def learned_throughput(reps=5):
     """Evaluations/s of DSF + learned correction vs Ewald (real-space error 1e-6) on the
     65,536-ion NaCl snapshot, on the GPU; accuracy is measured by the training script."""
@@ -196,6 +291,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph timing")
+    ap.add_argument("--no-drift", action="store_true", help="skip the 10^4-step energy-drift runs")
+    ap.add_argument("--no-richardson", action="store_true", help="skip the 48-system Richardson batch")
+    ap.add_argument("--no-gravity", action="store_true", help="skip the Plummer gravity section")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -226,6 +324,14 @@ def main():
         log(f"nacl: {nacl}")
     learned = None if args.no_learned else learned_throughput()
     log(f"learned: {learned}")
+    drift = None
+    if not args.no_drift:
+        drift = {p: drift_long(s.copy(), p) for p in ("mixed", "fp64")}
+        log(f"drift: {drift}")
+    rich = None if args.no_richardson else richardson_batch()
+    log(f"richardson: {rich}")
+    grav = None if args.no_gravity else gravity_plummer()
+    log(f"gravity: {grav}")
     alg_bytes = n * 148 + 4 * (mixed["list_entries"] / 2)
     achieved = alg_bytes / (mixed["kernel_ms"] * 1e-3) / 1e9
     cpu = None
@@ -258,6 +364,9 @@ def main():
         "energy_drift": {"mixed": mixed["energy_drift"], "fp64": fp64["energy_drift"] if fp64 else None},
         "nacl_65536_lj_dsf_steps_per_s": {k: v["steps_per_s"] for k, v in nacl.items()} if nacl else None,
         "learned_dsf_vs_ewald": learned,
+        "energy_drift_10k_steps": drift,
+        "richardson_48_uneven_lj": rich,
+        "plummer_262144_gravity": grav,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": FUSED_TRAFFIC_BYTES,
                      "kernel": "k_b_fused<float,LJ>", "kernel_ms": mixed["kernel_ms"],

@@ -14,6 +14,8 @@ timing candidates.
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 
 from . import _lib as L
@@ -98,7 +100,7 @@ class BrickEngine:
                 skins.append(dsf[0].skin)
             skin = min(skins)
             rc_list = max(r + skin for r in radii)
-            b = tuple(bricks[k]) if bricks is not None else default_bricks(s, rc_list)
+            b = tuple(bricks[k]) if bricks is not None and bricks[k] is not None else default_bricks(s, rc_list)
             self.bricks.append(b)
             bd = np.ascontiguousarray(b, np.int32)
             L.call("accu_bmd_set_system", h, k, len(s.species), L.ptr(mass), L.ptr(charge), self.dts[k],
@@ -185,7 +187,7 @@ class BrickEngine:
 
 
 def richardson(systems, make_forces, dt, duration_steps, precision="mixed", bricks=None, use_graph=True, block=10,
-               log=None):
+               log=None, threads=256):
     """timestep_convergence (accuracy.py:191-221) for many systems in one batched run.
 
     Every system is re-run from its current state at dt, dt/2 and dt/4 for
@@ -198,14 +200,16 @@ def richardson(systems, make_forces, dt, duration_steps, precision="mixed", bric
     dts = [dt / m for _, m in order]
     forces = [make_forces(r) for r in reps]
     br = None if bricks is None else [bricks[k] for k, _ in order]
-    eng = BrickEngine(reps, forces, VelocityVerlet(), dts, precision=precision, bricks=br)
+    eng = BrickEngine(reps, forces, VelocityVerlet(), dts, precision=precision, bricks=br, threads=threads)
     sched = (4, [S, S, 2 * S, 3 * S], [1.0] * 4, [1.0] * 4)
     left = duration_steps
+    t0 = time.perf_counter()
     while left > 0:
         blk = min(block, left) if use_graph else left
         eng.run(blk, record_every=10 ** 9, use_graph=use_graph and blk == block, sync=False, schedule=sched)
         left -= blk
     eng.sync()
+    eng.run_seconds = time.perf_counter() - t0
     eng.fetch(state=True)
     checks = []
     for k in range(S):

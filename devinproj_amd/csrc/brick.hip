@@ -93,12 +93,23 @@ __device__ int region_starts(const BBrick& b, const BRegion* __restrict__ reg, c
   return rs[R];
 }
 
-// Stage region atoms: lds[q] = pos + shift * L - org, species in w.
+// Nearest periodic image of coordinate d (brick-relative) to the centre c of its region
+// cell. Positions are wrapped into the box every step (as the reference does), so an atom
+// that crosses a box face between rebuilds keeps its cell but its wrapped coordinate jumps
+// by L; it is at most half a skin outside its cell, so the nearest image is unambiguous.
+__device__ __forceinline__ double near_image(double d, double c, double L) {
+  const double e = d - c;
+  return e > 0.5 * L ? d - L : (e < -0.5 * L ? d + L : d);
+}
+
+// Stage region atoms: lds[q] = pos + shift * L - org (the image nearest the region cell),
+// species in w.
 template <typename P>
 __device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
                              const int* __restrict__ start, const int* rs, int total, const double4* __restrict__ pos,
                              typename P4<P>::T* lds) {
   const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  const double h[3] = {s.L[0] / s.nc[0], s.L[1] / s.nc[1], s.L[2] / s.nc[2]};
   for (int q = threadIdx.x; q < total; q += blockDim.x) {
     int lo = 0, hi = R - 1;  // region cell r with rs[r] <= q < rs[r+1]
     while (lo < hi) {
@@ -110,9 +121,11 @@ __device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __re
     const int a = start[rc.gid] + (q - rs[lo]);
     const double4 x = pos[a];
     const int sx = rc.shift % 3 - 1, sy = (rc.shift / 3) % 3 - 1, sz = rc.shift / 9 - 1;
-    const double X = (x.x + sx * s.L[0]) - b.org[0];
-    const double Y = (x.y + sy * s.L[1]) - b.org[1];
-    const double Z = (x.z + sz * s.L[2]) - b.org[2];
+    // centre of region cell lo in brick-relative coordinates (cell 0 starts 1 + hd/2 cells below)
+    const int rz = lo % b.rd[2], ry = (lo / b.rd[2]) % b.rd[1], rx = lo / (b.rd[2] * b.rd[1]);
+    const double X = near_image((x.x + sx * s.L[0]) - b.org[0], (rx - 0.5 - 0.5 * b.hd[0]) * h[0], s.L[0]);
+    const double Y = near_image((x.y + sy * s.L[1]) - b.org[1], (ry - 0.5 - 0.5 * b.hd[1]) * h[1], s.L[1]);
+    const double Z = near_image((x.z + sz * s.L[2]) - b.org[2], (rz - 0.5 - 0.5 * b.hd[2]) * h[2], s.L[2]);
     typename P4<P>::T v;
     v.x = (P)X;
     v.y = (P)Y;
