@@ -14,6 +14,9 @@ csrc/mlp.hip, Adam on device), keeping Ewald as the judge.
 """
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 
 from . import _lib as L
@@ -149,6 +152,26 @@ class LearnedDSF(DampedShiftedCoulomb):
         return F, U
 
 
+NACL_CORRECTION = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "nacl_correction.npz")
+
+
+def save_correction(corr, path, meta=None):
+    """Parameters and geometry of a PairCorrection to an .npz (no pickled objects)."""
+    np.savez(path, params=corr.params, F0=corr.F0, rc=corr.rcut, r_on=corr.r_on, r_off=corr.r_off,
+             n_scale=corr.n_scale, meta=json.dumps(meta or {}))
+
+
+def load_correction(path=NACL_CORRECTION):
+    """PairCorrection from save_correction's file (allow_pickle=False). The default is the
+    network trained by train_nacl on the 65,536-ion NaCl melt (DSF alpha 0.2/A, rc 12 A),
+    whose training record is in `corr.meta`."""
+    d = np.load(path, allow_pickle=False)
+    corr = PairCorrection(float(d["rc"]), float(d["r_on"]), float(d["r_off"]), n_scale=float(d["n_scale"]),
+                          F0=float(d["F0"]), params=np.asarray(d["params"], np.float32))
+    corr.meta = json.loads(str(d["meta"]))
+    return corr
+
+
 def residual_targets(snapshots, dsf, ewald):
     """Ewald - DSF forces (newtons) for each snapshot system, computed on the GPU."""
     out = []
@@ -159,17 +182,22 @@ def residual_targets(snapshots, dsf, ewald):
     return out
 
 
-def train_correction(corr, snapshots, targets, epochs=10, lr=3e-3, seed=0, log=None):
-    """Adam over snapshots (one snapshot per step, shuffled each epoch). Returns the loss history."""
+def train_correction(corr, snapshots, targets, epochs=10, lr=3e-3, seed=0, log=None, lr_final=0.05):
+    """Adam over snapshots (one snapshot per step, shuffled each epoch), the learning rate
+    decaying from lr to lr * lr_final on a cosine over all steps. Returns the loss history
+    (mean per epoch)."""
     rng = np.random.default_rng(seed)
     th = L.torch()
     tgt = [th.from_numpy(np.ascontiguousarray(t, float)).cuda() for t in targets]
     hist = []
+    total, step = epochs * len(snapshots), 0
     for ep in range(epochs):
         tot = 0.0
         for k in rng.permutation(len(snapshots)):
             loss, _ = corr.grad(snapshots[k], tgt[k], want_grad=False)
-            corr.adam(lr)
+            frac = step / max(total - 1, 1)
+            corr.adam(lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + np.cos(np.pi * frac))))
+            step += 1
             tot += loss
         hist.append(tot / len(snapshots))
         if log:

@@ -21,3 +21,16 @@ def test_trained_correction_beats_plain_dsf_on_heldout_snapshots():
     s.pos = (s.pos + rng.normal(0, 0.3e-10, s.pos.shape)) % s.box
     F, _ = corr.compute(s)
     assert np.max(np.abs(F.sum(0))) < 1e-6 * np.max(np.abs(F))
+
+
+def test_shipped_correction_beats_plain_dsf_on_a_fresh_melt():
+    """The network in devinproj_amd/data (trained on the 65,536-ion melt) applied to a
+    different, smaller melt it never saw."""
+    corr = dp.learned.load_correction()
+    snaps = train_nacl.melt_and_sample((16, 16, 16), 2, 20, seed=21)
+    ew, dsf = train_nacl.ewald_1e6(), dp.DampedShiftedCoulomb(rcut=train_nacl.RC, alpha=train_nacl.ALPHA_DSF)
+    for s in snaps:
+        Fe, Fd, Fc = ew.compute(s)[0], dsf.compute(s)[0], corr.compute(s)[0]
+        e_dsf = np.sqrt(np.mean(np.sum((Fd - Fe) ** 2, 1)) / np.mean(np.sum(Fe ** 2, 1)))
+        e_corr = np.sqrt(np.mean(np.sum((Fd + Fc - Fe) ** 2, 1)) / np.mean(np.sum(Fe ** 2, 1)))
+        assert e_corr < 0.8 * e_dsf, (e_corr, e_dsf)

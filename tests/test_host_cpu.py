@@ -100,3 +100,19 @@ def test_workload_builders_match_oracle_builders():
     a, b = W.lj_reduced(500, 0.8, seed=4), B.lj_reduced(500, 0.8, seed=4)
     assert np.array_equal(a.pos, b.pos) and np.array_equal(a.vel, b.vel)
     assert W.uneven_batch() == B.uneven_batch()
+
+
+def test_shipped_correction_file_is_plain_data():
+    """The trained network ships as arrays only (no pickled objects), with its geometry and
+    the training record (held-out errors against Ewald)."""
+    import json
+    import os
+
+    import devinproj_amd
+    path = os.path.join(os.path.dirname(devinproj_amd.__file__), "data", "nacl_correction.npz")
+    d = np.load(path, allow_pickle=False)
+    assert d["params"].dtype == np.float32 and d["params"].shape == (4545,)
+    assert np.all(np.isfinite(d["params"]))
+    assert float(d["rc"]) == 12e-10 and float(d["r_on"]) < float(d["r_off"])
+    meta = json.loads(str(d["meta"]))
+    assert meta["rms_rel_error_dsf_plus_correction_vs_ewald"] < meta["rms_rel_error_dsf_vs_ewald"]
