@@ -175,6 +175,8 @@ SIGNATURES.update({
     "accu_bmd_sync": [P],
     "accu_bmd_get": [P, P, P, P, P, I, P, P],
     "accu_bmd_set_threads": [P, I],
+    "accu_bmd_set_snapshots": [P, I, I],
+    "accu_bmd_get_snapshots": [P, I, I, P, P, P, P],
     "accu_bmd_set_timing": [P, I],
     "accu_bmd_timing": [P, P, P],
 })

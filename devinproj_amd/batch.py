@@ -22,8 +22,9 @@ from . import _lib as L
 from .accuracy import richardson_check
 from .forces import DampedShiftedCoulomb, LennardJones, _erfc
 from .integrators import VelocityVerlet, Yoshida4
+from .simulation import Trajectory
 
-LOG_W = 8
+LOG_W = 12  # PE, KE, Px, Py, Pz, rebuilds, Lx, Ly, Lz, 0, 0, 0
 
 
 def default_bricks(s, rc_list, cap_atoms=1500):
@@ -47,10 +48,17 @@ class BrickEngine:
     over what the first build measures. A region or list that outgrows it later is
     reported as an error at fetch(). The LDS region sets how many workgroups share a CU,
     so a larger slack costs occupancy.
+
+    Every record (the set-up state and then every `record_every` steps of `run`) holds
+    each system's potential and kinetic energy, momentum and angular momentum, summed in
+    float64 on the device. With `snapshots` > 0, a device ring of that many frames also
+    keeps the positions and velocities of every `snapshot_every`-th record (the set-up
+    state is frame 0). `trajectory(k)` turns both into the reference's Trajectory
+    (simulation.py:14-39), which `devinproj_amd.io.save` writes in its .npz layout.
     """
 
     def __init__(self, systems, forces, integrator=None, dts=None, precision="fp64", bricks=None, slack=0.25,
-                 log_cap=20_000, threads=256):
+                 log_cap=20_000, threads=256, snapshots=0, snapshot_every=1):
         self.systems = list(systems)
         S = len(self.systems)
         if len(forces) != S:
@@ -115,8 +123,14 @@ class BrickEngine:
         self.log_cap = log_cap
         self.threads = threads
         L.call("accu_bmd_set_threads", h, int(threads))
+        self.snapshots, self.snapshot_every = int(snapshots), int(snapshot_every)
+        if self.snapshots:
+            L.call("accu_bmd_set_snapshots", h, self.snapshots, self.snapshot_every)
         L.call("accu_bmd_init", h, float(slack), int(log_cap))
         self.steps = 0
+        # simulated time of every record, per system (the set-up record is at t = 0)
+        self._clock = np.zeros(S)
+        self._rec_times = [self._clock.copy()]
 
     def run(self, steps, record_every=1, use_graph=False, sync=True, schedule=None):
         nt, active, wc, wn = schedule or self.ticks
@@ -125,6 +139,14 @@ class BrickEngine:
         n = np.ascontiguousarray(wn, float)
         L.call("accu_bmd_run", self._h, int(steps), int(nt), L.ptr(a), L.ptr(c), L.ptr(n), int(record_every),
                int(bool(use_graph)))
+        # time per coarse step of system k: its dt times the drift weights of the ticks it takes part in
+        S = len(self.systems)
+        per = np.array([self.dts[k] * sum(n[t] for t in range(nt) if k < a[t]) for k in range(S)])
+        if record_every > 0:
+            for c_ in range(1, steps + 1):
+                if c_ % record_every == 0 or c_ == steps:
+                    self._rec_times.append(self._clock + c_ * per)
+        self._clock = self._clock + steps * per
         self.steps += steps
         if sync:
             self.sync()
@@ -162,6 +184,54 @@ class BrickEngine:
                 self.forces_out.append(F[off:off + s.n])
                 off += s.n
         return self.logs
+
+    def snapshot_frames(self):
+        """(positions [F][N][3], velocities [F][N][3], record index [F]) of the stored
+        frames of the whole arena, in user order; frames beyond the ring are dropped."""
+        taken = L.C.c_int(0)
+        L.call("accu_bmd_get_snapshots", self._h, 0, 0, None, None, None, L.C.byref(taken))
+        nf = min(taken.value, self.snapshots)
+        N = sum(s.n for s in self.systems)
+        pos = np.empty((nf, N, 3))
+        vel = np.empty((nf, N, 3))
+        rec = np.empty(nf, np.int32)
+        if nf:
+            L.call("accu_bmd_get_snapshots", self._h, 0, nf, L.ptr(pos), L.ptr(vel), L.ptr(rec), L.C.byref(taken))
+        return pos, vel, rec
+
+    def trajectory(self, k=0):
+        """System k's records (and stored frames) as a Trajectory, after fetch().
+
+        Per record: t, kinetic, potential, momentum, angular_momentum from the device
+        log; internal, released and bath are 0 and mass, charge and counts constant
+        (no reactions on this path, simulation.py:76-93). Frames fill positions,
+        velocities and species; `frame_records` gives each frame's record index."""
+        s = self.systems[k]
+        lg = self.logs[k]
+        n = len(lg)
+        tr = Trajectory()
+        tr.t = [float(self._rec_times[r][k]) for r in range(n)]
+        tr.kinetic = lg[:, 1].tolist()
+        tr.potential = lg[:, 0].tolist()
+        tr.internal = [0.0] * n
+        tr.released = [0.0] * n
+        tr.momentum = [lg[r, 2:5].copy() for r in range(n)]
+        tr.angular_momentum = [lg[r, 6:9].copy() for r in range(n)]
+        tr.mass = [float(np.sum(s.masses))] * n
+        tr.charge = [float(np.sum(s.charges))] * n
+        tr.counts = [s.counts() for _ in range(n)]
+        tr.bath = [0.0] * n
+        tr.frame_records = []
+        if self.snapshots:
+            off = sum(x.n for x in self.systems[:k])
+            pos, vel, rec = self.snapshot_frames()
+            for f in range(len(rec)):
+                if 0 <= rec[f] < n:
+                    tr.positions.append(pos[f, off:off + s.n].copy())
+                    tr.velocities.append(vel[f, off:off + s.n].copy())
+                    tr.species.append(s.sp.copy())
+                    tr.frame_records.append(int(rec[f]))
+        return tr
 
     def energy_drift(self, k=0):
         lg = self.logs[k]

@@ -25,13 +25,15 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace accu {
 
 constexpr int kBT = 512;  // launch bound; the launch size is the runtime `threads`
 constexpr int kMaxRegionCells = 256;
-constexpr int kRecW = 8;  // PE, KE, Px, Py, Pz, rebuilds, 0, 0
+constexpr int kRecW = 12;  // PE, KE, Px, Py, Pz, rebuilds, Lx, Ly, Lz, 0, 0, 0
+constexpr int kRecP = 8;   // brick partials: PE, KE, Px, Py, Pz, Lx, Ly, Lz
 
 struct BSys {
   double L[3], inv[3];
@@ -102,12 +104,12 @@ __device__ __forceinline__ double near_image(double d, double c, double L) {
   return e > 0.5 * L ? d - L : (e < -0.5 * L ? d + L : d);
 }
 
-// Stage region atoms: lds[q] = pos + shift * L - org (the image nearest the region cell),
-// species in w.
+// Stage region atoms: lds[q] = (pos + shift * L - org) * scale (the image nearest the
+// region cell; scale is 1 or 1/sigma), species in w.
 template <typename P>
 __device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
                              const int* __restrict__ start, const int* rs, int total, const double4* __restrict__ pos,
-                             typename P4<P>::T* lds) {
+                             typename P4<P>::T* lds, double scale = 1.0) {
   const int R = b.rd[0] * b.rd[1] * b.rd[2];
   const double h[3] = {s.L[0] / s.nc[0], s.L[1] / s.nc[1], s.L[2] / s.nc[2]};
   for (int q = threadIdx.x; q < total; q += blockDim.x) {
@@ -127,9 +129,9 @@ __device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __re
     const double Y = near_image((x.y + sy * s.L[1]) - b.org[1], (ry - 0.5 - 0.5 * b.hd[1]) * h[1], s.L[1]);
     const double Z = near_image((x.z + sz * s.L[2]) - b.org[2], (rz - 0.5 - 0.5 * b.hd[2]) * h[2], s.L[2]);
     typename P4<P>::T v;
-    v.x = (P)X;
-    v.y = (P)Y;
-    v.z = (P)Z;
+    v.x = (P)(X * scale);
+    v.y = (P)(Y * scale);
+    v.z = (P)(Z * scale);
     v.w = (P)x.w;
     lds[q] = v;
   }
@@ -212,19 +214,19 @@ __device__ void record_systems(const BSys* __restrict__ sys, int S, const double
                                const int* __restrict__ nreb, double* red) {
   for (int si = 0; si < S; ++si) {
     const BSys& s = sys[si];
-    double acc[5] = {0, 0, 0, 0, 0};
+    double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
 #pragma unroll
-      for (int k = 0; k < 5; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * 5 + k];
-    block_sum<5>(acc, red);
+      for (int k = 0; k < kRecP; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * kRecP + k];
+    block_sum<kRecP>(acc, red);
     if (threadIdx.x == 0) {
       const int idx = log_count[si];
       if (idx < log_cap) {
         double* o = log + ((size_t)si * log_cap + idx) * kRecW;
         for (int k = 0; k < 5; ++k) o[k] = acc[k];
         o[5] = (double)nreb[si];
-        o[6] = 0.0;
-        o[7] = 0.0;
+        for (int k = 5; k < kRecP; ++k) o[k + 1] = acc[k];
+        for (int k = kRecP + 1; k < kRecW; ++k) o[k] = 0.0;
       }
       log_count[si] = idx + 1;
     }
@@ -396,7 +398,7 @@ __device__ __forceinline__ int rb_par(const RB& p, int si) { return p.par[si] ^ 
 __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   const int tid = threadIdx.x, bd = blockDim.x;
   if (p.rec && blockIdx.x == 0) {
-    __shared__ double red[5 * (kRB / 64)];
+    __shared__ double red[kRecP * (kRB / 64)];
     record_systems(p.sys, p.S, p.partials, p.log, p.log_cap, p.log_count, p.nreb, red);
   }
   if (p.flags[p.S] == 0 || p.ints[2] != 0) {
@@ -585,9 +587,20 @@ __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
 __global__ void k_b_record(const BSys* __restrict__ sys, const double* __restrict__ partials,
                            double* __restrict__ log, int log_cap, int* __restrict__ log_count,
                            const int* __restrict__ nreb) {
-  __shared__ double red[5 * 8];
+  __shared__ double red[kRecP * 8];
   const int si = blockIdx.x;
   record_systems(sys + si, 1, partials, log + (size_t)si * log_cap * kRecW, log_cap, log_count + si, nreb + si, red);
+}
+
+// After a snapshot frame: its record index (that of system 0, which every system shares:
+// records are taken at ticks that advance every system), then one more frame.
+__global__ void k_b_snap_bump(int* __restrict__ snap_count, int* __restrict__ snap_rec, int cap,
+                              const int* __restrict__ log_count) {
+  if (threadIdx.x == 0) {
+    const int f = snap_count[0];
+    if (f < cap) snap_rec[f] = log_count[0];
+    snap_count[0] = f + 1;
+  }
 }
 
 // a[0..n) = v (flags of every system at set-up, rebuild counts)
@@ -607,58 +620,62 @@ __device__ __forceinline__ double np_mod2(double x, double L) {
   return r;
 }
 
-// LJ pair of the engine (forces.py:127-141, full list so half the energy per visit):
-//   a = sr6^2, b = a - sr6; u += 2 eps (b - shift); coef = -24 eps (a + b) / r2.
-// The cutoff is an arithmetic 0/1 factor rather than a branch, so the eight pairs of a
-// group run as straight-line code: in = clamp((rc2 - r2) * ks, 0, 1) with
-// ks = 2^40 / rc2 (float32; 2^80 in float64) is exactly 1 for every r2 < rc2 (the
-// smallest nonzero relative gap is one ulp, 2^-24 or 2^-53) and exactly 0 for r2 >= rc2.
-// Tables: e2 = 2 eps, m24 = -24 eps, and rc2 = -1 (ks = 0 is not used: in = 0) where
-// eps = 0. x is capped at 1e6 so no power overflows for pairs the factor removes.
-template <typename P>
-__device__ __forceinline__ void lj_engine(P dx, P dy, P dz, P r2, P e2, P m24, P sig2, P rc2, P ks, P shift, P& fx,
-                                          P& fy, P& fz, P& u) {
-  P ir2, in;
-  if constexpr (sizeof(P) == 8) {
-    ir2 = rcp64(r2);
-    in = fmin(fmax((rc2 - r2) * ks, 0.0), 1.0);
-  } else {
-    ir2 = __builtin_amdgcn_rcpf(r2);
-    in = __builtin_amdgcn_fmed3f((rc2 - r2) * ks, 0.0f, 1.0f);
-  }
-  const P x = fmin(sig2 * ir2, P(1e6));
+// LJ pair of the engine (forces.py:127-141; full list, so half the energy per visit).
+// The cutoff is a select on 1/r2 rather than a branch, so the eight pairs of a group run
+// as straight-line code: w = 1/r2 for r2 < rc2 and w = 0 otherwise (also for the far
+// sentinel and, with rc2 = -1, for species pairs with eps = 0), so every term below is an
+// exact zero for pairs outside and nothing overflows for them. With x = sig2 w,
+// sr6 = x^3 and b = sr6^2 - sr6:
+//   f += (sr6^2 + b) w d            (times -24 eps),
+//   u += b, n += 1 for a pair inside (the energy, 2 eps (u - shift n), is formed per atom).
+// EPS: several species, so the pair's eps multiplies f, u and n (n then sums 2 eps over
+// the pairs inside); with one species the factors are applied once per atom.
+// UNIT_SIG: the staged coordinates are in units of sigma, so x = w.
+template <typename P, bool EN, bool EPS, bool UNIT_SIG>
+__device__ __forceinline__ void lj_engine(P dx, P dy, P dz, P r2, P e2, P m24, P sig2, P rc2, P& fx, P& fy, P& fz,
+                                          P& u, P& n) {
+  P ir2;
+  if constexpr (sizeof(P) == 8) ir2 = rcp64(r2);
+  else ir2 = __builtin_amdgcn_rcpf(r2);
+  const bool in = r2 < rc2;
+  const P w = in ? ir2 : P(0);
+  const P x = UNIT_SIG ? w : sig2 * w;
   const P sr6 = x * x * x;
-  const P a = sr6 * sr6;
-  const P b = a - sr6;
-  u = fma(e2 * in, b - shift, u);
-  const P coef = (m24 * in) * ((a + b) * ir2);
+  const P b = fma(sr6, sr6, -sr6);
+  P coef = fma(sr6, sr6, b) * w;
+  if constexpr (EPS) coef *= m24;
   fx = fma(coef, dx, fx);
   fy = fma(coef, dy, fy);
   fz = fma(coef, dz, fz);
+  if constexpr (EN) {
+    if constexpr (EPS) {
+      const P e = in ? e2 : P(0);
+      u = fma(e, b, u);
+      n += e;
+    } else {
+      u += b;
+      n += in ? P(1) : P(0);
+    }
+  }
 }
 
-template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN>
 __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P qi, const typename P4<P>::T& xi,
-                                       const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& cx, P& cy, P& cz,
-                                       P& cu) {
+                                       const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& n, P& cx, P& cy,
+                                       P& cz, P& cu) {
   const P dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
   const int tj = ONE_SP ? 0 : (int)xj.w;
   // list membership is decided at the build, so r^2 can use fused multiply-adds here
   const P r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-  if constexpr (sizeof(P) == 8) {
-    if constexpr (LJ_ON) {
-      const int pj = row + tj;
-      lj_engine<double>(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192 + pj], ljt[256 + pj], ljt[320],
-                        fx, fy, fz, u);
-    }
-    if constexpr (COUL_ON) coul_pair64(s.coul, dx, dy, dz, r2, qi * s.charge[tj], cx, cy, cz, cu);
-  } else {
-    if constexpr (LJ_ON) {
-      const int pj = row + tj;
-      lj_engine<float>(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj], ljt[192 + pj], ljt[256 + pj], ljt[320],
-                       fx, fy, fz, u);
-    }
-    if constexpr (COUL_ON)
+  if constexpr (LJ_ON) {
+    const int pj = row + tj;
+    lj_engine<P, EN, !ONE_SP, ONE_SP && !COUL_ON>(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj],
+                                                  ljt[192 + pj], fx, fy, fz, u, n);
+  }
+  if constexpr (COUL_ON) {
+    if constexpr (sizeof(P) == 8)
+      coul_pair64(s.coul, dx, dy, dz, r2, qi * s.charge[tj], cx, cy, cz, cu);
+    else
       coul_pair32((float)s.coul.alpha, (float)s.coul.rc, (float)s.coul.c, (float)s.coul.erc_rc, (float)s.coul.fshift,
                   s.coul.shifted, dx, dy, dz, r2, qi * (float)s.charge[tj], cx, cy, cz, cu);
   }
@@ -666,23 +683,29 @@ __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P q
 
 // One fused (sub)step of one brick: closing half kick with the new forces, then (for
 // bricks below drift_from) the next opening kick and drift into the other position
-// buffer, and the Verlet-distance test; with record, the brick's energy and momentum
-// partials.
-template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
+// buffer, and the Verlet-distance test; with EN (a recorded step), the brick's energy,
+// momentum and angular momentum partials (the pair energies are computed only then) and,
+// when snap is set, the positions and velocities of the recorded state in user order into
+// frame snap_count[0] of the snapshot ring (frames of N positions then N velocities).
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN>
 __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
                                                  const BRegion* __restrict__ reg, const int* __restrict__ start,
                                                  double4* __restrict__ posA, double4* __restrict__ posB,
                                                  const int* __restrict__ par, BState sta,
                                                  const int* __restrict__ spar, const int* __restrict__ nn,
                                                  const uint4* __restrict__ nbr, int stride, int cap, double wc,
-                                                 double wn, int drift_from, int record, int* __restrict__ flags,
-                                                 int S, double* __restrict__ partials, int* __restrict__ ints) {
+                                                 double wn, int drift_from, int* __restrict__ flags, int S,
+                                                 double* __restrict__ partials, int* __restrict__ ints,
+                                                 double* __restrict__ snap, int snap_cap,
+                                                 const int* __restrict__ snap_count, int N) {
   typedef typename P4<P>::T T4;
+  // one species and no Coulomb: positions are staged in units of sigma (x = 1/r2)
+  constexpr bool UNIT_SIG = LJ_ON && ONE_SP && !COUL_ON;
   const BBrick b = bricks[blockIdx.x];
   const BSys& s = sys[b.sys];
   __shared__ int rs[kMaxRegionCells + 1];
-  __shared__ P ljt[5 * 64 + 1];
-  __shared__ double red[5 * 8];
+  __shared__ P ljt[4 * 64];
+  __shared__ double red[kRecP * 8];
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T4* lds = reinterpret_cast<T4*>(lds_raw);
   const int si = b.sys;
@@ -693,28 +716,34 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   double4* __restrict__ Fst = sta.F[sp];
   const double4* __restrict__ ref = sta.ref[sp];
   const int* __restrict__ cell = sta.cell[sp];
+  // species pair tables: 2 eps, -24 eps, sigma^2, and rc^2 (-1 where eps = 0: no pair is inside)
+  const bool lj0 = LJ_ON && s.lj_eps[0] > 0.0 && s.lj_rc2[0] > 0.0;
+  const double unit = UNIT_SIG && lj0 ? sqrt(s.lj_sig2[0]) : 1.0;
   if constexpr (LJ_ON) {
     const int m = s.ns * s.ns;
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
       const double e = s.lj_eps[k];
+      const bool on = e > 0.0 && s.lj_rc2[k] > 0.0;
       ljt[k] = (P)(2.0 * e);
       ljt[64 + k] = (P)(-24.0 * e);
       ljt[128 + k] = (P)s.lj_sig2[k];
-      const bool on = e > 0.0 && s.lj_rc2[k] > 0.0;
-      ljt[192 + k] = on ? (P)s.lj_rc2[k] : (P)-1;
-      // 2^40 / rc2 in float32, 2^80 / rc2 in float64 (see lj_engine)
-      ljt[256 + k] = on ? (P)((sizeof(P) == 8 ? 0x1p80 : 0x1p40) / s.lj_rc2[k]) : (P)0;
+      ljt[192 + k] = on ? (P)(UNIT_SIG ? s.lj_rc2[k] / s.lj_sig2[k] : s.lj_rc2[k]) : (P)-1;
     }
-    if (threadIdx.x == 0) ljt[320] = (P)s.lj_shift;
   }
   const int total = region_starts(b, reg, start, rs);
   // uniform: an overflowing region (or a failed rebuild) skips the work; the host reports it
   const bool ok = total + 1 <= cap && ints[2] == 0;
   if (!ok && threadIdx.x == 0 && total + 1 > cap) atomicMax(&ints[1], total + 1);
-  double acc[5] = {0, 0, 0, 0, 0};
+  double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool moved_far = false;
+  // snapshot frame of this recorded step (uniform; none when the ring is full)
+  double* frame = nullptr;
+  if (EN && snap) {
+    const int f = snap_count[0];
+    if (f < snap_cap) frame = snap + (size_t)f * N * 6;
+  }
   if (ok) {
-    stage_region<P>(b, s, reg, start, rs, total, pos, lds);
+    stage_region<P>(b, s, reg, start, rs, total, pos, lds, 1.0 / unit);
     if (threadIdx.x == 0) {  // far sentinel: the padding entries of the lists point here
       T4 far;
       far.x = (P)1e15;
@@ -735,8 +764,8 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
       const int row = ti * s.ns;
       const P qi = (P)s.charge[ti];
       const int ng = (nn[a] + 7) >> 3;
-      double Fx = 0, Fy = 0, Fz = 0, U = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
-      P fx = 0, fy = 0, fz = 0, u = 0, cx = 0, cy = 0, cz = 0, cu = 0;
+      double Fx = 0, Fy = 0, Fz = 0, U = 0, Nin = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
+      P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
       // groups of 8 neighbours; the next group's indices load while this one computes
       uint4 cur = make_uint4(0u, 0u, 0u, 0u);
       if (ng > 0) cur = nbr[a];
@@ -748,18 +777,29 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
         for (int h2 = 0; h2 < 2; ++h2) {
           const T4 x0 = lds[w[2 * h2] & 0xffffu], x1 = lds[w[2 * h2] >> 16];
           const T4 x2 = lds[w[2 * h2 + 1] & 0xffffu], x3 = lds[w[2 * h2 + 1] >> 16];
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, n, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, n, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, n, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, n, cx, cy, cz, cu);
         }
         if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 8 neighbours
-          Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
-          fx = fy = fz = u = cx = cy = cz = cu = 0;
+          Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+          fx = fy = fz = u = n = cx = cy = cz = cu = 0;
         }
         cur = nxt;
       }
-      Fx += fx; Fy += fy; Fz += fz; U += u; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+      Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+      if constexpr (LJ_ON) {
+        if constexpr (ONE_SP) {  // one species: -24 eps (per sigma for unit-sigma coordinates), 2 eps
+          const double f = -24.0 * s.lj_eps[0] / unit;
+          Fx *= f;
+          Fy *= f;
+          Fz *= f;
+          U = 2.0 * s.lj_eps[0] * (U - s.lj_shift * Nin);
+        } else {
+          U -= s.lj_shift * Nin;
+        }
+      }
       if constexpr (COUL_ON) {
         const double q = s.charge[ti];
         CU -= q * q * s.coul.self;
@@ -777,12 +817,26 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
         v.y = v.y + h_cur * ay;
         v.z = v.z + h_cur * az;
       }
-      if (record) {
+      if constexpr (EN) {  // system.py:106-114: KE, P = sum m v, L = sum x cross (m v)
+        const double4 x = pos[a];
+        const double px = m * v.x, py = m * v.y, pz = m * v.z;
         acc[0] += U;
         acc[1] += 0.5 * m * ((v.x * v.x + v.z * v.z) + v.y * v.y);
-        acc[2] += m * v.x;
-        acc[3] += m * v.y;
-        acc[4] += m * v.z;
+        acc[2] += px;
+        acc[3] += py;
+        acc[4] += pz;
+        acc[5] += x.y * pz - x.z * py;
+        acc[6] += x.z * px - x.x * pz;
+        acc[7] += x.x * py - x.y * px;
+        if (frame) {
+          const size_t u = (size_t)sta.perm[sp][a];
+          frame[3 * u] = x.x;
+          frame[3 * u + 1] = x.y;
+          frame[3 * u + 2] = x.z;
+          frame[3 * ((size_t)N + u)] = v.x;
+          frame[3 * ((size_t)N + u) + 1] = v.y;
+          frame[3 * ((size_t)N + u) + 2] = v.z;
+        }
       }
       Fst[a] = make_double4(Fx, Fy, Fz, 0.0);
       if (do_drift) {
@@ -813,10 +867,10 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
     atomicOr(&flags[si], 1);
     atomicOr(&flags[S], 1);
   }
-  if (record) {
-    block_sum<5>(acc, red);
+  if constexpr (EN) {
+    block_sum<kRecP>(acc, red);
     if (threadIdx.x == 0)
-      for (int k2 = 0; k2 < 5; ++k2) partials[(size_t)blockIdx.x * 5 + k2] = acc[k2];
+      for (int k2 = 0; k2 < kRecP; ++k2) partials[(size_t)blockIdx.x * kRecP + k2] = acc[k2];
   }
 }
 
@@ -882,6 +936,21 @@ __global__ void k_b_scatter_user(int natoms, int which, const double4* __restric
 
 // ------------------------------------------------------------------ host engine
 
+// The dynamic-LDS limit of every instantiation of the fused kernel.
+template <typename P, typename F>
+void set_fused_attrs(F&& set) {
+  auto one = [&](auto on, auto en) {
+    constexpr bool O = decltype(on)::value, E = decltype(en)::value;
+    set((const void*)k_b_fused<P, true, true, O, E>);
+    set((const void*)k_b_fused<P, true, false, O, E>);
+    set((const void*)k_b_fused<P, false, true, O, E>);
+  };
+  one(std::true_type{}, std::true_type{});
+  one(std::true_type{}, std::false_type{});
+  one(std::false_type{}, std::true_type{});
+  one(std::false_type{}, std::false_type{});
+}
+
 struct SysSpec {
   int natoms = 0;
   double box[3] = {0, 0, 0};
@@ -930,6 +999,13 @@ struct BMD {
   double* log = nullptr;
   int log_cap = 0;
   double* partials = nullptr;
+  // snapshot ring: frames of N positions then N velocities (user order) of every
+  // snap_every-th record; snap_count[0] = frames taken, snap_rec[f] = record index of frame f
+  double* snap = nullptr;
+  int snap_cap = 0, snap_every = 0;
+  int* snap_count = nullptr;
+  int* snap_rec = nullptr;
+  long long rec_enq = 0;  // records enqueued so far (the set-up record is record 0)
   int cap_fused = 0, cap_build = 0;  // region capacity (atoms) for the LDS
   int threads = 256;                 // workgroup size of the per-brick kernels (64..512)
   bool ready = false;
@@ -939,12 +1015,13 @@ struct BMD {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<long long> graph_key;
+  long long graph_recs = 0;  // records in one replay of the graph
 
   void release() {
     void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, spar, sta.vel[0],
                   sta.vel[1], sta.F[0], sta.F[1], sta.ref[0], sta.ref[1], sta.perm[0], sta.perm[1], sta.cell[0],
                   sta.cell[1], count, start, tmp, slot, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
-                  bar, bsum, tphase};
+                  bar, bsum, tphase, snap, snap_count, snap_rec};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1020,22 +1097,18 @@ struct BMD {
     hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(threads), lds_build(), st, p);
   }
 
-  template <typename P, bool ONE>
-  void fused_t(int nb, double wc, double wn, int drift_from, int rec, bool lj, bool co) {
+  template <typename P, bool ONE, bool EN>
+  void fused_t(int nb, double wc, double wn, int drift_from, bool lj, bool co, bool take) {
     const size_t sh = lds_fused();
-
-    if (lj && co)
-      hipLaunchKernelGGL((k_b_fused<P, true, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
-                         pos, pos_b, par, sta, spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
-                         flags, S, partials, ints);
-    else if (lj)
-      hipLaunchKernelGGL((k_b_fused<P, true, false, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
-                         pos, pos_b, par, sta, spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
-                         flags, S, partials, ints);
-    else
-      hipLaunchKernelGGL((k_b_fused<P, false, true, ONE>), dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start,
-                         pos, pos_b, par, sta, spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, rec,
-                         flags, S, partials, ints);
+    double* sn = take ? snap : nullptr;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(threads), sh, st, d_sys, d_bricks, d_reg, start, pos, pos_b, par, sta,
+                         spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, flags, S, partials, ints, sn,
+                         snap_cap, snap_count, N);
+    };
+    if (lj && co) go(k_b_fused<P, true, true, ONE, EN>);
+    else if (lj) go(k_b_fused<P, true, false, ONE, EN>);
+    else go(k_b_fused<P, false, true, ONE, EN>);
   }
 
   bool any_lj = false, any_coul = false, one_sp = true;
@@ -1043,6 +1116,9 @@ struct BMD {
   // One fused (sub)step of the first `nb` bricks; bricks below drift_from also drift
   // (their systems' buffer parity flips with the rebuild launch that follows).
   void launch_fused(int nb, double wc, double wn, int drift_from, int rec) {
+    // a recorded step of every system takes a snapshot on every snap_every-th record
+    const bool take = rec && snap && nb == nbricks && rec_enq % snap_every == 0;
+    if (rec) ++rec_enq;
     if (timing) {
       if (ev_used + 2 > ev.size()) {
         hipEvent_t a, b;
@@ -1053,13 +1129,20 @@ struct BMD {
       }
       (void)hipEventRecord(ev[ev_used], st);
     }
+    const bool l = any_lj, c = any_coul;
     if (precision == ACCU_FP64) {
-      if (one_sp) fused_t<double, true>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
-      else fused_t<double, false>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
+      if (one_sp) rec ? fused_t<double, true, true>(nb, wc, wn, drift_from, l, c, take)
+                      : fused_t<double, true, false>(nb, wc, wn, drift_from, l, c, false);
+      else rec ? fused_t<double, false, true>(nb, wc, wn, drift_from, l, c, take)
+               : fused_t<double, false, false>(nb, wc, wn, drift_from, l, c, false);
     } else {
-      if (one_sp) fused_t<float, true>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
-      else fused_t<float, false>(nb, wc, wn, drift_from, rec, any_lj, any_coul);
+      if (one_sp) rec ? fused_t<float, true, true>(nb, wc, wn, drift_from, l, c, take)
+                      : fused_t<float, true, false>(nb, wc, wn, drift_from, l, c, false);
+      else rec ? fused_t<float, false, true>(nb, wc, wn, drift_from, l, c, take)
+               : fused_t<float, false, false>(nb, wc, wn, drift_from, l, c, false);
     }
+    // the frame is complete: note its record index (the record is written after this) and advance the ring
+    if (take) hipLaunchKernelGGL(k_b_snap_bump, dim3(1), dim3(64), 0, st, snap_count, snap_rec, snap_cap, log_count);
     if (timing) {
       (void)hipEventRecord(ev[ev_used + 1], st);
       ev_used += 2;
@@ -1366,7 +1449,7 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   if (rc) return rc;
   HIP_TRY(hipMalloc(&m.count, (m.ncells + 1) * sizeof(int)));
   HIP_TRY(hipMalloc(&m.start, (m.ncells + 1) * sizeof(int)));
-  HIP_TRY(hipMalloc(&m.partials, (size_t)m.nbricks * 5 * sizeof(double)));
+  HIP_TRY(hipMalloc(&m.partials, (size_t)m.nbricks * kRecP * sizeof(double)));
   m.log_cap = std::max(log_cap, 4);
   HIP_TRY(hipMalloc(&m.log, (size_t)m.S * m.log_cap * kRecW * sizeof(double)));
   m.stride = (m.N + 63) / 64 * 64;
@@ -1444,18 +1527,8 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   auto set_attr = [&](const void* f) {
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_fused());
   };
-  set_attr((const void*)k_b_fused<double, true, true, true>);
-  set_attr((const void*)k_b_fused<double, true, false, true>);
-  set_attr((const void*)k_b_fused<double, false, true, true>);
-  set_attr((const void*)k_b_fused<double, true, true, false>);
-  set_attr((const void*)k_b_fused<double, true, false, false>);
-  set_attr((const void*)k_b_fused<double, false, true, false>);
-  set_attr((const void*)k_b_fused<float, true, true, true>);
-  set_attr((const void*)k_b_fused<float, true, false, true>);
-  set_attr((const void*)k_b_fused<float, false, true, true>);
-  set_attr((const void*)k_b_fused<float, true, true, false>);
-  set_attr((const void*)k_b_fused<float, true, false, false>);
-  set_attr((const void*)k_b_fused<float, false, true, false>);
+  set_fused_attrs<double>(set_attr);
+  set_fused_attrs<float>(set_attr);
   m.launch_fused(m.nbricks, 0.0, 0.0, 0, 1);
   m.launch_record();
   LAUNCH_CHECK();
@@ -1489,6 +1562,8 @@ int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const d
     key.push_back((long long)(wc[t] * 1e15));
     key.push_back((long long)(wn[t] * 1e15));
   }
+  // which records of the block take snapshots is fixed at capture: replay only in the same phase
+  key.push_back(m.snap ? m.rec_enq % m.snap_every : -1);
   if (!(m.gexec && key == m.graph_key)) {
     if (m.gexec) HIP_TRY(hipGraphExecDestroy(m.gexec));
     if (m.graph) HIP_TRY(hipGraphDestroy(m.graph));
@@ -1496,10 +1571,13 @@ int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const d
     m.graph = nullptr;
     const bool tm = m.timing;
     m.timing = false;
+    const long long rec0 = m.rec_enq;
     HIP_TRY(hipStreamBeginCapture(m.st, hipStreamCaptureModeThreadLocal));
     m.enqueue(coarse, nticks, active, wc, wn, record_every);
     hipError_t e = hipStreamEndCapture(m.st, &m.graph);
     m.timing = tm;
+    m.graph_recs = m.rec_enq - rec0;
+    m.rec_enq = rec0;
     if (e != hipSuccess) {
       set_error("graph capture failed: %s", hipGetErrorString(e));
       return ACCU_ERUNTIME;
@@ -1508,6 +1586,7 @@ int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const d
     m.graph_key = key;
   }
   HIP_TRY(hipGraphLaunch(m.gexec, m.st));
+  m.rec_enq += m.graph_recs;
   return ACCU_OK;
 }
 
@@ -1607,6 +1686,61 @@ int accu_bmd_set_threads(accu_bmd* h, int threads) {
     m.graph = nullptr;
     m.graph_key.clear();
   }
+  return ACCU_OK;
+}
+
+int accu_bmd_set_snapshots(accu_bmd* h, int cap_frames, int every) {
+  ARG_CHECK(h, "null handle");
+  ARG_CHECK(cap_frames >= 0 && every >= 1, "bad snapshot ring (capacity %d, every %d)", cap_frames, every);
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  HIP_TRY(hipStreamSynchronize(m.st));
+  for (void* p : {(void*)m.snap, (void*)m.snap_count, (void*)m.snap_rec})
+    if (p) HIP_TRY(hipFree(p));
+  m.snap = nullptr;
+  m.snap_count = m.snap_rec = nullptr;
+  m.snap_cap = 0;
+  m.snap_every = every;
+  if (cap_frames > 0) {
+    HIP_TRY(hipMalloc(&m.snap, (size_t)cap_frames * m.N * 6 * sizeof(double)));
+    HIP_TRY(hipMalloc(&m.snap_count, 64));
+    HIP_TRY(hipMalloc(&m.snap_rec, (size_t)cap_frames * sizeof(int)));
+    HIP_TRY(hipMemset(m.snap_count, 0, 64));
+    HIP_TRY(hipMemset(m.snap_rec, 0xff, (size_t)cap_frames * sizeof(int)));
+    m.snap_cap = cap_frames;
+  }
+  // the record phase restarts at the next record; a captured graph holds the old ring
+  m.rec_enq = 0;
+  if (m.gexec) {
+    (void)hipGraphExecDestroy(m.gexec);
+    (void)hipGraphDestroy(m.graph);
+    m.gexec = nullptr;
+    m.graph = nullptr;
+    m.graph_key.clear();
+  }
+  return ACCU_OK;
+}
+
+int accu_bmd_get_snapshots(accu_bmd* h, int first, int count, double* pos_host, double* vel_host, int* rec_host,
+                           int* taken_host) {
+  ARG_CHECK(h, "null handle");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  int taken = 0;
+  if (m.snap_count) {
+    HIP_TRY(hipMemcpyAsync(&taken, m.snap_count, sizeof(int), hipMemcpyDeviceToHost, m.st));
+    HIP_TRY(hipStreamSynchronize(m.st));
+  }
+  if (taken_host) *taken_host = taken;
+  if (count == 0) return ACCU_OK;
+  const int stored = std::min(taken, m.snap_cap);
+  ARG_CHECK(first >= 0 && count > 0 && first + count <= stored, "frames [%d, %d) not stored (%d stored)", first,
+            first + count, stored);
+  const size_t n3 = (size_t)m.N * 3;
+  for (int f = 0; f < count; ++f) {
+    const double* src = m.snap + (size_t)(first + f) * n3 * 2;
+    if (pos_host) HIP_TRY(hipMemcpy(pos_host + f * n3, src, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (vel_host) HIP_TRY(hipMemcpy(vel_host + f * n3, src + n3, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (rec_host) HIP_TRY(hipMemcpy(rec_host, m.snap_rec + first, count * sizeof(int), hipMemcpyDeviceToHost));
   return ACCU_OK;
 }
 

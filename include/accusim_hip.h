@@ -141,7 +141,9 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap);
 int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active_host, const double* wc_host,
                  const double* wn_host, int record_every, int use_graph);
 int accu_bmd_sync(accu_bmd* h);
-/* state in user order; log_host [nsys][log_max][8]; stats [nsys][4] (rebuilds, records);
+/* state in user order; log_host [nsys][log_max][12] = PE, KE, Px, Py, Pz, rebuilds so
+ * far, Lx, Ly, Lz (L = sum x cross m v over wrapped positions, system.py:112-114), 0, 0, 0
+ * per record (Simulation.record, simulation.py:76-93); stats [nsys][4] (rebuilds, records);
  * info [16] = max neighbours, capacity, region capacity, region overflow, list entries,
  * then the last rebuild's phase ends in device wall-clock ticks after its start
  * ([5] binning, [6] block sums, [7] scan, [8] grouping by cell, [9] in-cell order and
@@ -150,6 +152,18 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
                  long long* stats_host, long long* info_host);
 /* workgroup size of the per-brick kernels (multiple of 64 in [64, 512]; default 256) */
 int accu_bmd_set_threads(accu_bmd* h, int threads);
+/* device-resident trajectory frames (the positions/velocities that Simulation.record
+ * appends, simulation.py:90-93, stored as io.py:19-25 saves them): a ring of cap_frames
+ * frames of [N][3] positions then [N][3] velocities in user order, written by the fused
+ * step of every `every`-th record (record 0 is the set-up state when this is called before
+ * accu_bmd_init). cap_frames = 0 frees the ring. Frames beyond the capacity are counted,
+ * not stored. */
+int accu_bmd_set_snapshots(accu_bmd* h, int cap_frames, int every);
+/* frames [first, first + count) to pos_host / vel_host ([count][N][3], either may be
+ * NULL) and their record indices to rec_host [count] (may be NULL); taken_host: frames
+ * taken so far (including those beyond the capacity) */
+int accu_bmd_get_snapshots(accu_bmd* h, int first, int count, double* pos_host, double* vel_host, int* rec_host,
+                           int* taken_host);
 int accu_bmd_set_timing(accu_bmd* h, int enable);
 int accu_bmd_timing(accu_bmd* h, double* total_ms_host, long long* launches_host);
 
