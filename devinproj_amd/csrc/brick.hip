@@ -32,6 +32,7 @@ namespace accu {
 
 constexpr int kBT = 512;  // launch bound; the launch size is the runtime `threads`
 constexpr int kMaxRegionCells = 256;
+constexpr int kMaxHomeCells = 128;
 constexpr int kRecW = 12;  // PE, KE, Px, Py, Pz, rebuilds, Lx, Ly, Lz, 0, 0, 0
 constexpr int kRecP = 8;   // brick partials: PE, KE, Px, Py, Pz, Lx, Ly, Lz
 
@@ -80,67 +81,100 @@ struct P4<float> {
 
 // ------------------------------------------------------------------ region staging
 
-// Region cell starts (exclusive prefix of counts) into rs[0..R]; returns the total.
-__device__ int region_starts(const BBrick& b, const BRegion* __restrict__ reg, const int* __restrict__ start, int* rs) {
-  const int R = b.rd[0] * b.rd[1] * b.rd[2];
-  for (int r = threadIdx.x; r < R; r += blockDim.x) {
-    const int c = reg[b.reg_off + r].gid;
-    rs[r + 1] = start[c + 1] - start[c];
+// Exclusive prefix of v over the block; total receives the block sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  if (threadIdx.x == 0) rs[0] = 0;
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int r = 1; r <= R; ++r) rs[r] += rs[r - 1];
+  if (lane == 63) wsum[w] = x;
   __syncthreads();
-  return rs[R];
-}
-
-// Nearest periodic image of coordinate d (brick-relative) to the centre c of its region
-// cell. Positions are wrapped into the box every step (as the reference does), so an atom
-// that crosses a box face between rebuilds keeps its cell but its wrapped coordinate jumps
-// by L; it is at most half a skin outside its cell, so the nearest image is unambiguous.
-__device__ __forceinline__ double near_image(double d, double c, double L) {
-  const double e = d - c;
-  return e > 0.5 * L ? d - L : (e < -0.5 * L ? d + L : d);
-}
-
-// Stage region atoms: lds[q] = (pos + shift * L - org) * scale (the image nearest the
-// region cell; scale is 1 or 1/sigma), species in w.
-template <typename P>
-__device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
-                             const int* __restrict__ start, const int* rs, int total, const double4* __restrict__ pos,
-                             typename P4<P>::T* lds, double scale = 1.0) {
-  const int R = b.rd[0] * b.rd[1] * b.rd[2];
-  const double h[3] = {s.L[0] / s.nc[0], s.L[1] / s.nc[1], s.L[2] / s.nc[2]};
-  for (int q = threadIdx.x; q < total; q += blockDim.x) {
-    int lo = 0, hi = R - 1;  // region cell r with rs[r] <= q < rs[r+1]
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (rs[mid] <= q) lo = mid;
-      else hi = mid - 1;
-    }
-    const BRegion rc = reg[b.reg_off + lo];
-    const int a = start[rc.gid] + (q - rs[lo]);
-    const double4 x = pos[a];
-    const int sx = rc.shift % 3 - 1, sy = (rc.shift / 3) % 3 - 1, sz = rc.shift / 9 - 1;
-    // centre of region cell lo in brick-relative coordinates (cell 0 starts 1 + hd/2 cells below)
-    const int rz = lo % b.rd[2], ry = (lo / b.rd[2]) % b.rd[1], rx = lo / (b.rd[2] * b.rd[1]);
-    const double X = near_image((x.x + sx * s.L[0]) - b.org[0], (rx - 0.5 - 0.5 * b.hd[0]) * h[0], s.L[0]);
-    const double Y = near_image((x.y + sy * s.L[1]) - b.org[1], (ry - 0.5 - 0.5 * b.hd[1]) * h[1], s.L[1]);
-    const double Z = near_image((x.z + sz * s.L[2]) - b.org[2], (rz - 0.5 - 0.5 * b.hd[2]) * h[2], s.L[2]);
-    typename P4<P>::T v;
-    v.x = (P)(X * scale);
-    v.y = (P)(Y * scale);
-    v.z = (P)(Z * scale);
-    v.w = (P)x.w;
-    lds[q] = v;
+  int woff = 0, tot = 0;
+  for (int k = 0; k < nw; ++k) {
+    woff += k < w ? wsum[k] : 0;
+    tot += wsum[k];
   }
+  total = tot;
+  return woff + x - v;
 }
 
 __device__ __forceinline__ int home_region(const BBrick& b, int local) {
   // home cell local index (z fastest) -> region cell of the same cell
   const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
   return ((lx + 1) * b.rd[1] + (ly + 1)) * b.rd[2] + (lz + 1);
+}
+
+// Region set-up of one brick: cell starts rs[0..R] (block scan of the cell counts) and
+// the region slot of each home cell's first atom (home atom a of home cell `local` sits
+// in slot hself[local] + a). Returns the number of region atoms.
+__device__ int region_setup(const BBrick& b, const BRegion* __restrict__ reg, const int* __restrict__ start, int* rs,
+                            int* wsum, int* hself) {
+  const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  int carry = 0;
+  for (int r0 = 0; r0 < R; r0 += blockDim.x) {  // uniform trip count
+    const int r = r0 + threadIdx.x;
+    int cnt = 0;
+    if (r < R) {
+      const int c = reg[b.reg_off + r].gid;
+      cnt = start[c + 1] - start[c];
+    }
+    int tot;
+    const int ex = block_excl_scan(cnt, wsum, tot);
+    if (r < R) rs[r] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) rs[R] = carry;
+  __syncthreads();
+  for (int l = threadIdx.x; l < b.nhome; l += blockDim.x) hself[l] = rs[home_region(b, l)] - start[b.cell0 + l];
+  __syncthreads();
+  return carry;
+}
+
+// Stage region atoms: lds[q] = (pos - org) * scale with species in w, where pos is taken
+// at the periodic image nearest the centre of its region cell. That image is the one the
+// region's cell shift names; positions are also wrapped into the box every step (as the
+// reference does), so an atom that crosses a box face between rebuilds keeps its cell
+// while its wrapped coordinate jumps by L; it is at most half a skin outside its cell and
+// cells are at most L/3 wide, so the nearest image is unambiguous. Each region cell is
+// staged by a group of blockDim/R threads (consecutive slots, coalesced reads), which
+// needs neither a slot search nor any per-slot integer division.
+template <typename P>
+__device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
+                             const int* __restrict__ start, const int* rs, const double4* __restrict__ pos,
+                             typename P4<P>::T* lds, double scale = 1.0) {
+  const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  const int G = max(1, (int)blockDim.x / R);  // threads per region cell
+  const int per = (int)blockDim.x / G;         // region cells per pass
+  const int sub = threadIdx.x % G;
+  for (int r = threadIdx.x / G; r < R && threadIdx.x < per * G; r += per) {
+    const int q0 = rs[r], q1 = rs[r + 1];
+    const int base = start[reg[b.reg_off + r].gid] - q0;
+    const int rr[3] = {r / (b.rd[2] * b.rd[1]), (r / b.rd[2]) % b.rd[1], r % b.rd[2]};
+    double cen[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)  // region cell 0 starts 1 + hd/2 cells below the brick centre
+      cen[d] = (rr[d] - 0.5 - 0.5 * b.hd[d]) * (s.L[d] / s.nc[d]);
+    for (int q = q0 + sub; q < q1; q += G) {
+      const double4 x = pos[base + q];
+      const double xs[3] = {x.x, x.y, x.z};
+      double X[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double e = xs[d] - b.org[d];
+        X[d] = fma(-s.L[d], rint((e - cen[d]) * s.inv[d]), e);
+      }
+      typename P4<P>::T v;
+      v.x = (P)(X[0] * scale);
+      v.y = (P)(X[1] * scale);
+      v.z = (P)(X[2] * scale);
+      v.w = (P)x.w;
+      lds[q] = v;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ rebuild (one launch)
@@ -281,27 +315,6 @@ __device__ __forceinline__ int block_sum_int(int v, int* wsum) {
   return t;
 }
 
-// Exclusive prefix of v over the block; total receives the block sum.
-__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  __syncthreads();
-  if (lane == 63) wsum[w] = x;
-  __syncthreads();
-  int woff = 0, tot = 0;
-  for (int k = 0; k < nw; ++k) {
-    woff += k < w ? wsum[k] : 0;
-    tot += wsum[k];
-  }
-  total = tot;
-  return woff + x - v;
-}
-
 // Each system's current positions live in posA or posB according to par[system]
 // (double buffering per system, so systems can advance at different rates).
 __device__ __forceinline__ const double4* cur_buf(const double4* A, const double4* B, const int* par, int s) {
@@ -314,15 +327,14 @@ __device__ __forceinline__ const double4* cur_buf(const double4* A, const double
 // rc_list (a superset is harmless: forces apply their own cutoffs), so the test runs
 // in float32 on brick-relative coordinates with a 1e-5 relative margin on rc_list^2,
 // and region cells whose box is beyond rc_list from the atom are skipped.
-__device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs, int& mymax) {
+__device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs, int* hself, int* wsum, int& mymax) {
   const BSys& s = p.sys[b.sys];
-  const int total = region_starts(b, p.reg, p.start, rs);
+  const int total = region_setup(b, p.reg, p.start, rs, wsum, hself);
   if (total + 1 > p.cap) {
     if (threadIdx.x == 0) atomicMax(&p.ints[1], total + 1);
-    __syncthreads();  // rs is rewritten by the next brick of this block
     return;
   }
-  stage_region<float>(b, s, p.reg, p.start, rs, total, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
+  stage_region<float>(b, s, p.reg, p.start, rs, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
   __syncthreads();
   const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
@@ -333,8 +345,7 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
   const int a0 = p.start[b.cell0], a1 = p.start[b.cell0 + b.nhome];
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
     const int local = cell[a] - b.cell0;
-    const int rh = home_region(b, local);
-    const int self = rs[rh] + (a - p.start[b.cell0 + local]);
+    const int self = hself[local] + a;
     const float4 xi = lds32[self];
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
     // 8 entries per 16-byte group: each hit shifts the 128-bit register down by 16 bits
@@ -575,9 +586,11 @@ __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   const BBrick b = p.bricks[blockIdx.x];
   if (!p.bflags[b.sys]) return;
   __shared__ int rs[kMaxRegionCells + 1];
+  __shared__ int hself[kMaxHomeCells];
+  __shared__ int wsum[kBT / 64];
   extern __shared__ __align__(16) float4 lds32[];
   int mymax = 0;
-  build_brick(p, b, lds32, rs, mymax);
+  build_brick(p, b, lds32, rs, hself, wsum, mymax);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
   if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&p.ints[0], mymax);
@@ -704,6 +717,8 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   const BBrick b = bricks[blockIdx.x];
   const BSys& s = sys[b.sys];
   __shared__ int rs[kMaxRegionCells + 1];
+  __shared__ int hself[kMaxHomeCells];
+  __shared__ int wsum[kBT / 64];
   __shared__ P ljt[4 * 64];
   __shared__ double red[kRecP * 8];
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -730,7 +745,7 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
       ljt[192 + k] = on ? (P)(UNIT_SIG ? s.lj_rc2[k] / s.lj_sig2[k] : s.lj_rc2[k]) : (P)-1;
     }
   }
-  const int total = region_starts(b, reg, start, rs);
+  const int total = region_setup(b, reg, start, rs, wsum, hself);
   // uniform: an overflowing region (or a failed rebuild) skips the work; the host reports it
   const bool ok = total + 1 <= cap && ints[2] == 0;
   if (!ok && threadIdx.x == 0 && total + 1 > cap) atomicMax(&ints[1], total + 1);
@@ -743,7 +758,7 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
     if (f < snap_cap) frame = snap + (size_t)f * N * 6;
   }
   if (ok) {
-    stage_region<P>(b, s, reg, start, rs, total, pos, lds, 1.0 / unit);
+    stage_region<P>(b, s, reg, start, rs, pos, lds, 1.0 / unit);
     if (threadIdx.x == 0) {  // far sentinel: the padding entries of the lists point here
       T4 far;
       far.x = (P)1e15;
@@ -757,8 +772,7 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
     const bool do_drift = (int)blockIdx.x < drift_from;
     const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
     for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
-      const int local = cell[a] - b.cell0;
-      const int self = rs[home_region(b, local)] + (a - start[b.cell0 + local]);
+      const int self = hself[cell[a] - b.cell0] + a;
       const T4 xi = lds[self];
       const int ti = ONE_SP ? 0 : (int)xi.w;
       const int row = ti * s.ns;
@@ -1254,7 +1268,8 @@ static int build_tables(BMD& m) {
             b.org[d] = (lo[d] + 0.5 * b.hd[d]) * (sp.box[d] / s.nc[d]);
           }
           b.nhome = b.hd[0] * b.hd[1] * b.hd[2];
-          ARG_CHECK(b.rd[0] * b.rd[1] * b.rd[2] <= kMaxRegionCells, "brick region too large");
+          ARG_CHECK(b.rd[0] * b.rd[1] * b.rd[2] <= kMaxRegionCells && b.nhome <= kMaxHomeCells,
+                    "brick region too large");
           for (int lx = 0; lx < b.hd[0]; ++lx)
             for (int ly = 0; ly < b.hd[1]; ++ly)
               for (int lz = 0; lz < b.hd[2]; ++lz) {
@@ -1502,7 +1517,7 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   const size_t lds_max = prop.sharedMemPerBlock;
   m.cap_build = cap;
   m.cap_fused = cap;
-  ARG_CHECK(m.lds_build() + 4096 <= lds_max && m.lds_fused() + 4096 <= lds_max && cap < 65536,
+  ARG_CHECK(m.lds_build() + 6144 <= lds_max && m.lds_fused() + 6144 <= lds_max && cap < 65536,
             "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
   (void)hipFuncSetAttribute((const void*)k_b_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
   // neighbour capacity: measure with one group of 8, then size
