@@ -700,8 +700,17 @@ __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P q
 // momentum and angular momentum partials (the pair energies are computed only then) and,
 // when snap is set, the positions and velocities of the recorded state in user order into
 // frame snap_count[0] of the snapshot ring (frames of N positions then N velocities).
+// Waves per SIMD the compiler must leave room for: the float32 one-species LJ step fits
+// 80 VGPRs without spilling, so three 512-thread bricks share a CU (as their LDS regions
+// do); the other forms keep the compiler's own allocation (bounding them spills).
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
+constexpr int fused_waves() {
+  return sizeof(P) == 4 && LJ_ON && !COUL_ON && ONE_SP ? 6 : 1;
+}
+
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN>
-__global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
+__global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(fused_waves<P, LJ_ON, COUL_ON, ONE_SP>())))
+k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
                                                  const BRegion* __restrict__ reg, const int* __restrict__ start,
                                                  double4* __restrict__ posA, double4* __restrict__ posB,
                                                  const int* __restrict__ par, BState sta,
@@ -749,7 +758,11 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
   // uniform: an overflowing region (or a failed rebuild) skips the work; the host reports it
   const bool ok = total + 1 <= cap && ints[2] == 0;
   if (!ok && threadIdx.x == 0 && total + 1 > cap) atomicMax(&ints[1], total + 1);
-  double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // EN: the brick's partials, summed by thread 0 over the passes of the atom loop, so
+  // that no per-thread sums stay live across the pair loop
+  __shared__ double tot[kRecP];
+  if (EN && threadIdx.x == 0)
+    for (int k2 = 0; k2 < kRecP; ++k2) tot[k2] = 0.0;
   bool moved_far = false;
   // snapshot frame of this recorded step (uniform; none when the ring is full)
   double* frame = nullptr;
@@ -771,7 +784,10 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
     const double h_cur = 0.5 * wc * s.dt, h_next = 0.5 * wn * s.dt, dt_next = wn * s.dt;
     const bool do_drift = (int)blockIdx.x < drift_from;
     const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
-    for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+    for (int pass = a0; pass < a1; pass += blockDim.x) {  // uniform trip count
+      const int a = pass + threadIdx.x;
+      double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (a < a1) {
       const int self = hself[cell[a] - b.cell0] + a;
       const T4 xi = lds[self];
       const int ti = ONE_SP ? 0 : (int)xi.w;
@@ -875,6 +891,12 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
         moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
       }
       vel[a] = v;
+      }
+      if constexpr (EN) {
+        block_sum<kRecP>(acc, red);
+        if (threadIdx.x == 0)
+          for (int k2 = 0; k2 < kRecP; ++k2) tot[k2] += acc[k2];
+      }
     }
   }
   if (__any(moved_far) && (threadIdx.x & 63) == 0) {
@@ -882,9 +904,8 @@ __global__ void __launch_bounds__(kBT) k_b_fused(const BSys* __restrict__ sys, c
     atomicOr(&flags[S], 1);
   }
   if constexpr (EN) {
-    block_sum<kRecP>(acc, red);
     if (threadIdx.x == 0)
-      for (int k2 = 0; k2 < kRecP; ++k2) partials[(size_t)blockIdx.x * kRecP + k2] = acc[k2];
+      for (int k2 = 0; k2 < kRecP; ++k2) partials[(size_t)blockIdx.x * kRecP + k2] = tot[k2];
   }
 }
 
