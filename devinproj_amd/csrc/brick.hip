@@ -505,7 +505,8 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   if (blockIdx.x == 0 && tid == 0) p.tphase[4] = wall_clock64();
   // 4. rank each cell's atoms by original index and gather each atom's state to its
   //    sorted slot in the other buffers; the list reference positions are the gathered
-  //    positions. When both cells of a pair have at most 32 atoms each half-wave takes one; otherwise
+  //    positions. Forces are not gathered: a rebuild always sits between a drift and
+  //    the fused step that recomputes every force of the system before any use. When both cells of a pair have at most 32 atoms each half-wave takes one; otherwise
   //    the whole wave takes the two in turn (shuffle rank up to 64 atoms, a rank against
   //    the cell's atoms in memory beyond that). Control flow stays wave-uniform.
   {
@@ -517,7 +518,6 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       (pc ? p.posA : p.posB)[t] = x;
       p.st.ref[sc ^ 1][t] = x;
       p.st.vel[sc ^ 1][t] = p.st.vel[sc][v];
-      p.st.F[sc ^ 1][t] = p.st.F[sc][v];
       p.st.perm[sc ^ 1][t] = p.st.perm[sc][v];
       p.st.cell[sc ^ 1][t] = c;
     };
