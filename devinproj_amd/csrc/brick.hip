@@ -35,6 +35,7 @@ constexpr int kMaxRegionCells = 256;
 constexpr int kMaxHomeCells = 128;
 constexpr int kRecW = 12;  // PE, KE, Px, Py, Pz, rebuilds, Lx, Ly, Lz, 0, 0, 0
 constexpr int kRecP = 8;   // brick partials: PE, KE, Px, Py, Pz, Lx, Ly, Lz
+constexpr int kSplit = 8;  // fused step: lanes per atom in a short last pass
 
 struct BSys {
   double L[3], inv[3];
@@ -785,23 +786,32 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
     const bool do_drift = (int)blockIdx.x < drift_from;
     const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
     for (int pass = a0; pass < a1; pass += blockDim.x) {  // uniform trip count
-      const int a = pass + threadIdx.x;
+      // a pass with at most half as many atoms as threads (the last pass of a brick with
+      // more atoms than threads, or a small brick) gives each atom L lanes (a power of two
+      // up to kSplit), which take its neighbour groups in turn and then sum their partial
+      // forces in a fixed shuffle order, so that the pass costs about 1/L of a full one
+      int L = 1;
+      while (2 * L <= kSplit && (a1 - pass) * 2 * L <= (int)blockDim.x) L *= 2;
+      const bool split = L > 1;
+      const int a = pass + (int)threadIdx.x / L;
+      const int sub = (int)threadIdx.x % L, gstep = L;
       double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
+      double Fx = 0, Fy = 0, Fz = 0, U = 0, Nin = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
+      int ti = 0;
       if (a < a1) {
       const int self = hself[cell[a] - b.cell0] + a;
       const T4 xi = lds[self];
-      const int ti = ONE_SP ? 0 : (int)xi.w;
+      ti = ONE_SP ? 0 : (int)xi.w;
       const int row = ti * s.ns;
       const P qi = (P)s.charge[ti];
       const int ng = (nn[a] + 7) >> 3;
-      double Fx = 0, Fy = 0, Fz = 0, U = 0, Nin = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
       P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
       // groups of 8 neighbours; the next group's indices load while this one computes
       uint4 cur = make_uint4(0u, 0u, 0u, 0u);
-      if (ng > 0) cur = nbr[a];
-      for (int g = 0; g < ng; ++g) {
+      if (sub < ng) cur = nbr[(size_t)sub * stride + a];
+      for (int g = sub; g < ng; g += gstep) {
         uint4 nxt = cur;
-        if (g + 1 < ng) nxt = nbr[(size_t)(g + 1) * stride + a];
+        if (g + gstep < ng) nxt = nbr[(size_t)(g + gstep) * stride + a];
         const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll
         for (int h2 = 0; h2 < 2; ++h2) {
@@ -819,6 +829,27 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
         cur = nxt;
       }
       Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+      }
+      if (split) {
+#pragma unroll
+        for (int o = 1; o < kSplit; o <<= 1) {
+          if (o >= L) break;
+          Fx += __shfl_xor(Fx, o, 64);
+          Fy += __shfl_xor(Fy, o, 64);
+          Fz += __shfl_xor(Fz, o, 64);
+          if constexpr (EN) {
+            U += __shfl_xor(U, o, 64);
+            Nin += __shfl_xor(Nin, o, 64);
+          }
+          if constexpr (COUL_ON) {
+            Cx += __shfl_xor(Cx, o, 64);
+            Cy += __shfl_xor(Cy, o, 64);
+            Cz += __shfl_xor(Cz, o, 64);
+            if constexpr (EN) CU += __shfl_xor(CU, o, 64);
+          }
+        }
+      }
+      if (a < a1 && sub == 0) {
       if constexpr (LJ_ON) {
         if constexpr (ONE_SP) {  // one species: -24 eps (per sigma for unit-sigma coordinates), 2 eps
           const double f = -24.0 * s.lj_eps[0] / unit;
