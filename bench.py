@@ -35,10 +35,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the mixed fused kernel on this workload (the tuned tile, 3x3x3
-# bricks at 512 threads): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the
-# factors being what the two counters read for a 1 GiB device copy (bench.py
-# --copy-probe 1024); the summary with its commands is profiles/k_b_fused_lj_argon_1048576.md
-FUSED_TRAFFIC_BYTES = 5.265e8
+# bricks at 512 threads, energies recorded): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# per dispatch, the factors being what the two counters read for a 1 GiB device copy
+# (bench.py --copy-probe 1024); the summary with its commands is
+# profiles/k_b_fused_lj_argon_1048576.md
+FUSED_TRAFFIC_BYTES = 5.333e8
 DT = 10e-15
 # brick tiles: the autotuner's choice on MI355X for each workload and precision
 # (python -m devinproj_amd.autotune writes devinproj_amd/data/tiles_mi355x.json); the
