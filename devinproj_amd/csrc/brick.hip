@@ -250,9 +250,23 @@ __device__ void record_systems(const BSys* __restrict__ sys, int S, const double
   for (int si = 0; si < S; ++si) {
     const BSys& s = sys[si];
     double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
+    // two bricks per thread and pass: their loads (which miss the L2, the partials being
+    // written by the previous launch) are in flight together; the sum order stays fixed
+    const double* __restrict__ ps = partials + (size_t)s.brick_off * kRecP;
+    for (int b = threadIdx.x; b < s.nbricks; b += 2 * blockDim.x) {
+      const int b2 = b + blockDim.x < s.nbricks ? b + blockDim.x : b;
+      const double w2 = b + blockDim.x < s.nbricks ? 1.0 : 0.0;
+      double v[kRecP], u[kRecP];
 #pragma unroll
-      for (int k = 0; k < kRecP; ++k) acc[k] += partials[(size_t)(s.brick_off + b) * kRecP + k];
+      for (int k = 0; k < kRecP; ++k) {
+        v[k] = ps[(size_t)b * kRecP + k];
+        u[k] = ps[(size_t)b2 * kRecP + k];
+      }
+#pragma unroll
+      for (int k = 0; k < kRecP; ++k) acc[k] += v[k];
+#pragma unroll
+      for (int k = 0; k < kRecP; ++k) acc[k] = fma(w2, u[k], acc[k]);
+    }
     block_sum<kRecP>(acc, red);
     if (threadIdx.x == 0) {
       const int idx = log_count[si];
@@ -1216,7 +1230,7 @@ struct BMD {
   }
 
   void launch_record() {
-    hipLaunchKernelGGL(k_b_record, dim3(S), dim3(256), 0, st, d_sys, partials, log, log_cap, log_count, nreb);
+    hipLaunchKernelGGL(k_b_record, dim3(S), dim3(512), 0, st, d_sys, partials, log, log_cap, log_count, nreb);
   }
 
   // Enqueue `coarse` coarse steps of `nticks` ticks; at tick t the first active[t]
