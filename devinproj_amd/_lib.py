@@ -179,4 +179,7 @@ SIGNATURES.update({
     "accu_bmd_get_snapshots": [P, I, I, P, P, P, P],
     "accu_bmd_set_timing": [P, I],
     "accu_bmd_timing": [P, P, P],
+    "accu_bmd_set_integrator": [P, I, D, D, D, C.c_ulonglong],
+    "accu_bmd_set_diffusion": [P, I, I, P],
+    "accu_bmd_get_thermostat": [P, P],
 })

@@ -51,6 +51,13 @@ def drift_rows(tr, s, forces, integ):
     else:
         why = "thermostat" if integ.thermostat else "dissipative force"
         checks.append(Check("energy", NA, None, f"not conserved by design ({why})"))
+    bath = getattr(tr, "bath", [])
+    if hasattr(integ, "bath_energy") and all(f.conservative for f in forces) and len(bath) == len(tr.t):
+        # accuracy.py:81-85
+        Ex = E + np.array(bath)
+        dx = float(np.max(np.abs(Ex - Ex[0])) / scale)
+        checks.append(Check("extended energy", _grade(dx, 1e-5, 1e-3), dx,
+                            "Nose-Hoover conserved quantity: E + thermostat energy"))
     P = np.array(tr.momentum)
     pscale = max(float(np.sum(s.masses[:, None] * np.abs(s.vel))), 1e-300)
     if not external and not walls and not integ.thermostat:
@@ -73,6 +80,19 @@ def drift_rows(tr, s, forces, integ):
     qscale = max(float(np.sum(np.abs(s.charges))), 1e-300)
     dq = float(np.max(np.abs(Qc - Qc[0])) / qscale)
     checks.append(Check("charge", _grade(dq, 1e-12, 1e-9), dq, "total charge"))
+    from .integrators import Langevin, NoseHoover
+    if isinstance(integ, (Langevin, NoseHoover)) and len(tr.t) >= 4:
+        # accuracy.py:112-123: mean temperature over the second half of the records
+        from .units import k_B
+        half = len(tr.t) // 2
+        kin = np.asarray(tr.kinetic[half:])
+        n = np.array([sum(c.values()) for c in tr.counts[half:]], dtype=float)
+        dof = s.dim * np.maximum(n - 1, 1) if isinstance(integ, NoseHoover) else s.dim * np.maximum(n, 1)
+        T_meas = float(np.mean(2 * kin / (dof * k_B)))
+        dev = abs(T_meas - integ.T) / integ.T
+        tol = max(0.02, 3 * np.sqrt(2 / (s.dim * max(s.n, 1))))
+        checks.append(Check("thermostat", PASS if dev <= tol else (WARN if dev <= 0.1 else FAIL), dev,
+                            f"mean T over 2nd half {T_meas:.4g} K vs target {integ.T:.4g} K"))
     return checks
 
 

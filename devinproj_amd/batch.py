@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib as L
 from .accuracy import richardson_check
 from .forces import DampedShiftedCoulomb, LennardJones, _erfc
-from .integrators import VelocityVerlet, Yoshida4
+from .integrators import Brownian, Langevin, NoseHoover, VelocityVerlet, Yoshida4
 from .simulation import Trajectory
 
 LOG_W = 12  # PE, KE, Px, Py, Pz, rebuilds, Lx, Ly, Lz, 0, 0, 0
@@ -67,13 +67,27 @@ class BrickEngine:
             dts = [dts] * S
         self.dts = [float(d) for d in dts]
         self.integrator = integrator or VelocityVerlet()
-        if isinstance(self.integrator, Yoshida4):
+        ig = self.integrator
+        # integrator mode of the device engine (accu_bmd_set_integrator)
+        if isinstance(ig, Yoshida4):
             w = Yoshida4.weights()
             self.ticks = (3, [S, S, S], [w[0], w[1], w[2]], [w[1], w[2], w[0]])
-        elif isinstance(self.integrator, VelocityVerlet):
+            self._mode = (0, 0.0, 0.0, 0.0, 0)
+        elif isinstance(ig, VelocityVerlet):
             self.ticks = (1, [S], [1.0], [1.0])
+            self._mode = (0, 0.0, 0.0, 0.0, 0)
+        elif isinstance(ig, (Langevin, NoseHoover, Brownian)):
+            self.ticks = (1, [S], [1.0], [1.0])
+            if isinstance(ig, Langevin):
+                self._mode = (1, ig.T, ig.gamma, 0.0, ig.seed)
+            elif isinstance(ig, Brownian):
+                self._mode = (2, ig.T, 0.0, 0.0, ig.seed)
+            else:
+                if ig.Q is not None:
+                    raise ValueError("the device Nose-Hoover starts from a fresh thermostat (xi = eta = 0)")
+                self._mode = (3, ig.T, 0.0, ig.tau, 0)
         else:
-            raise ValueError("the brick engine runs VelocityVerlet or Yoshida4")
+            raise ValueError("the brick engine runs VelocityVerlet, Yoshida4, Langevin, NoseHoover or Brownian")
         self.precision = precision
         L.lib()
         for s in self.systems:
@@ -120,6 +134,12 @@ class BrickEngine:
                 f = dsf[0]
                 erc = float(_erfc(np.array(f.alpha * f.rcut)))
                 L.call("accu_bmd_set_dsf", h, k, float(f.alpha), float(f.rcut), erc, float(f.k))
+            if self._mode[0] == 2:
+                D = np.ascontiguousarray([x.diffusion for x in s.species], float)
+                L.call("accu_bmd_set_diffusion", h, k, len(s.species), L.ptr(D))
+        mode, T, gamma, tau, seed = self._mode
+        L.call("accu_bmd_set_integrator", h, mode, float(T), float(gamma), float(tau),
+               L.C.c_ulonglong(int(seed) & 0xFFFFFFFFFFFFFFFF))
         self.log_cap = log_cap
         self.threads = threads
         L.call("accu_bmd_set_threads", h, int(threads))
@@ -220,7 +240,7 @@ class BrickEngine:
         tr.mass = [float(np.sum(s.masses))] * n
         tr.charge = [float(np.sum(s.charges))] * n
         tr.counts = [s.counts() for _ in range(n)]
-        tr.bath = [0.0] * n
+        tr.bath = lg[:, 9].tolist()  # Nose-Hoover bath energy (0 for the other integrators)
         tr.frame_records = []
         if self.snapshots:
             off = sum(x.n for x in self.systems[:k])
@@ -232,6 +252,13 @@ class BrickEngine:
                     tr.species.append(s.sp.copy())
                     tr.frame_records.append(int(rec[f]))
         return tr
+
+    def thermostat_state(self):
+        """Nose-Hoover state per system: rows (xi, eta, Q, the velocity scales of the last
+        two half steps, k2)."""
+        out = np.zeros((len(self.systems), 6))
+        L.call("accu_bmd_get_thermostat", self._h, L.ptr(out))
+        return out
 
     def energy_drift(self, k=0):
         lg = self.logs[k]

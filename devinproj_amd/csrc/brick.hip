@@ -47,6 +47,7 @@ struct BSys {
   int cellmap_off;
   int ns;
   double mass[kMaxSpecies], charge[kMaxSpecies];
+  double diff[kMaxSpecies];  // Brownian diffusion coefficient per species
   int has_lj, has_coul;
   double lj_eps[kMaxSpecies * kMaxSpecies], lj_sig2[kMaxSpecies * kMaxSpecies], lj_rc2[kMaxSpecies * kMaxSpecies];
   double lj_shift;
@@ -229,6 +230,7 @@ struct RB {
   int* flags;   // [S] per system + [S] any, set by the fused step
   int* bflags;  // the same, as handed to the list launch
   int* nreb;
+  long long* ndrift;  // per system: drifts done (the stochastic integrators' noise counter)
   int S, N, ncells, nbricks;
   int flip_n;    // systems [0, flip_n) drifted: their parity flips with this launch
   int bin_only;  // 1: stop after the scan (set-up measures the regions)
@@ -429,7 +431,10 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   }
   if (p.flags[p.S] == 0 || p.ints[2] != 0) {
     if (blockIdx.x == 0) {  // nothing to rebuild: commit the flip, the list launch exits
-      for (int s = tid; s < p.flip_n; s += bd) p.par[s] ^= 1;
+      for (int s = tid; s < p.flip_n; s += bd) {
+        p.par[s] ^= 1;
+        p.ndrift[s] += 1;
+      }
       if (tid == 0) p.bflags[p.S] = 0;
     }
     return;
@@ -580,6 +585,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     for (int s = tid; s < p.S; s += bd) {
       const int f = p.flags[s];
       p.par[s] ^= (s < p.flip_n ? 1 : 0) ^ (f ? 1 : 0);
+      if (s < p.flip_n) p.ndrift[s] += 1;
       if (f) p.spar[s] ^= 1;
       p.bflags[s] = f;
       if (f) {
@@ -954,10 +960,70 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
   }
 }
 
-// Opening half kick + drift of the first (sub)step for the atoms of the active systems.
+// Integrator of the engine (every system of an arena uses the same one).
+enum { kVV = 0, kLangevin = 1, kBrownian = 2, kNoseHoover = 3 };
+struct BInteg {
+  int mode;
+  double gamma;  // Langevin friction (1/s)
+  double T;      // bath temperature (K)
+  double tau;    // Nose-Hoover period (s)
+  unsigned long long seed;
+};
+constexpr double kBoltzmann = 1.380649e-23;
+
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC11): 10 rounds of two 32x32->64
+// multiplies with Weyl key increments.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned k0, unsigned k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const unsigned long long p0 = 0xD2511F53ull * c.x, p1 = 0xCD9E8D57ull * c.z;
+    c = make_uint4((unsigned)(p1 >> 32) ^ c.y ^ k0, (unsigned)p1, (unsigned)(p0 >> 32) ^ c.w ^ k1, (unsigned)p0);
+  }
+  return c;
+}
+
+// Uniform in (0, 1) from two words: ((a >> 5) 2^26 + (b >> 6) + 0.5) 2^-53.
+__device__ __forceinline__ double u53(unsigned a, unsigned b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * 1.1102230246251565e-16;
+}
+
+// Three standard normals of atom `id` at drift number `ctr` (Box-Muller on 4 uniforms).
+__device__ __forceinline__ void noise3(unsigned long long seed, long long ctr, unsigned id, double& n0, double& n1,
+                                       double& n2) {
+  const unsigned k0 = (unsigned)seed, k1 = (unsigned)(seed >> 32);
+  const unsigned c0 = (unsigned)ctr, c1 = (unsigned)((unsigned long long)ctr >> 32);
+  const uint4 w = philox4x32_10(make_uint4(c0, c1, id, 0u), k0, k1);
+  const uint4 z = philox4x32_10(make_uint4(c0, c1, id, 1u), k0, k1);
+  const double ua = u53(w.x, w.y), ub = u53(w.z, w.w), uc = u53(z.x, z.y);
+  const double r0 = sqrt(-2.0 * log(ua)), r1 = sqrt(-2.0 * log(uc));
+  const double t0 = 2.0 * M_PI * ub;
+  n0 = r0 * cos(t0);
+  n1 = r0 * sin(t0);
+  n2 = r1 * cos(2.0 * M_PI * u53(z.z, z.w));
+}
+
+// The drift of one step for the atoms of the active systems (the first step of a run,
+// and every step of the thermostatted integrators, whose fused launch stops after the
+// closing kick), in the reference's operation order:
+//   velocity Verlet (integrators.py:19-31): v += dt/2 a; x += dt v
+//   Langevin BAOAB (integrators.py:60-71): v += dt/2 a; x += dt/2 v;
+//       v = c1 v + sqrt((1 - c1^2) kB T / m) xi with c1 = exp(-gamma dt); x += dt/2 v
+//   Brownian (integrators.py:250-257): dx = D F / (kB T) dt + sqrt(2 D dt) xi; x += dx;
+//       v = dx / dt
+//   Nose-Hoover (integrators.py:104-112): v *= scale (the thermostat half steps, from
+//       k_b_nh); v += dt/2 a; x += dt v
+// then the periodic wrap and the Verlet-distance test. With drift == 0 only the
+// Nose-Hoover scale is applied (the last half step of a run). xi: Philox normals of the
+// atom's user index at the system's drift number.
 __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ sysid, int natoms,
                           double4* __restrict__ posA, double4* __restrict__ posB, const int* __restrict__ par,
-                          BState sta, const int* __restrict__ spar, double wn, int* __restrict__ flags, int S) {
+                          BState sta, const int* __restrict__ spar, double wn, int* __restrict__ flags, int S,
+                          BInteg ig, const long long* __restrict__ ndrift, const double* __restrict__ nh,
+                          int drift) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   bool moved_far = false;
   int si = 0;
@@ -970,35 +1036,148 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
     double4* vel = sta.vel[sp];
     const double4* ref = sta.ref[sp];
     const double4 x0 = pos[a];
-    const double m = s.mass[(int)x0.w];
+    const int ti = (int)x0.w;
+    const double m = s.mass[ti];
     const double4 f = sta.F[sp][a];
     const double h = 0.5 * wn * s.dt, dtn = wn * s.dt;
     double4 v = vel[a], x = x0;
+    double n0 = 0, n1 = 0, n2 = 0;
+    if (ig.mode == kLangevin || ig.mode == kBrownian) noise3(ig.seed, ndrift[si], (unsigned)sta.perm[sp][a], n0, n1, n2);
     {
 #pragma clang fp contract(off)
-      v.x = v.x + h * (f.x / m);
-      v.y = v.y + h * (f.y / m);
-      v.z = v.z + h * (f.z / m);
-      x.x = x.x + dtn * v.x;
-      x.y = x.y + dtn * v.y;
-      x.z = x.z + dtn * v.z;
+      if (ig.mode == kNoseHoover) {
+        const double s1 = nh[6 * si + 3], s2 = nh[6 * si + 4];
+        v.x = v.x * s1 * s2;
+        v.y = v.y * s1 * s2;
+        v.z = v.z * s1 * s2;
+      }
+      if (drift) {
+        if (ig.mode == kBrownian) {
+          const double D = s.diff[ti], kT = kBoltzmann * ig.T, sq = sqrt(2.0 * D * dtn);
+          const double dx = D * f.x / kT * dtn + sq * n0;
+          const double dy = D * f.y / kT * dtn + sq * n1;
+          const double dz = D * f.z / kT * dtn + sq * n2;
+          x.x = x.x + dx;
+          x.y = x.y + dy;
+          x.z = x.z + dz;
+          v.x = dx / dtn;
+          v.y = dy / dtn;
+          v.z = dz / dtn;
+        } else {
+          v.x = v.x + h * (f.x / m);
+          v.y = v.y + h * (f.y / m);
+          v.z = v.z + h * (f.z / m);
+          if (ig.mode == kLangevin) {
+            const double hd = 0.5 * dtn;
+            x.x = x.x + hd * v.x;
+            x.y = x.y + hd * v.y;
+            x.z = x.z + hd * v.z;
+            const double c1 = exp(-ig.gamma * dtn);
+            const double sg = sqrt((1.0 - c1 * c1) * kBoltzmann * ig.T / m);
+            v.x = c1 * v.x + sg * n0;
+            v.y = c1 * v.y + sg * n1;
+            v.z = c1 * v.z + sg * n2;
+            x.x = x.x + hd * v.x;
+            x.y = x.y + hd * v.y;
+            x.z = x.z + hd * v.z;
+          } else {
+            x.x = x.x + dtn * v.x;
+            x.y = x.y + dtn * v.y;
+            x.z = x.z + dtn * v.z;
+          }
+        }
+      }
     }
-    x.x = np_mod2(x.x, s.L[0]);
-    x.y = np_mod2(x.y, s.L[1]);
-    x.z = np_mod2(x.z, s.L[2]);
     vel[a] = v;
-    pos_next[a] = x;
-    const double4 r = ref[a];
-    const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
-    const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
-    const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
-    moved_far = !(r2_np(d0, d1, d2) < s.half_skin2);
-    if (moved_far) {
-      atomicOr(&flags[si], 1);
-      atomicOr(&flags[S], 1);
+    if (drift) {
+      x.x = np_mod2(x.x, s.L[0]);
+      x.y = np_mod2(x.y, s.L[1]);
+      x.z = np_mod2(x.z, s.L[2]);
+      pos_next[a] = x;
+      const double4 r = ref[a];
+      const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+      const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+      const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+      moved_far = !(r2_np(d0, d1, d2) < s.half_skin2);
+      if (moved_far) {
+        atomicOr(&flags[si], 1);
+        atomicOr(&flags[S], 1);
+      }
     }
   }
 }
+
+// The Nose-Hoover half steps of one system (k_b_nh), in the reference's operation order.
+__device__ void nh_update(const BSys& s, const BInteg& ig, const double* acc, double* st, int end, int start, int rec,
+                          double* log, int log_cap, int* log_count, int nreb) {
+#pragma clang fp contract(off)
+  double xi = st[0], eta = st[1], Q = st[2];
+  const double kT = kBoltzmann * ig.T, nf = 3.0 * (s.natoms - 1 > 1 ? s.natoms - 1 : 1), dt = s.dt;
+  double k2 = !end && st[5] >= 0.0 ? st[5] : 2.0 * acc[1];
+  double s1 = 1.0, s2 = 1.0;
+  auto half = [&]() {
+    if (Q == 0.0) Q = nf * kT * (ig.tau * ig.tau);
+    xi = xi + 0.25 * dt * (k2 - nf * kT) / Q;
+    const double sc = exp(-0.5 * dt * xi);
+    eta = eta + 0.5 * dt * xi;
+    xi = xi + 0.25 * dt * (k2 * (sc * sc) - nf * kT) / Q;
+    k2 = k2 * (sc * sc);
+    return sc;
+  };
+  if (end) {
+    s1 = half();
+    if (rec) {
+      const int idx = log_count[0];
+      if (idx < log_cap) {
+        double* o = log + (size_t)idx * kRecW;
+        o[0] = acc[0];
+        o[1] = acc[1] * (s1 * s1);
+        for (int k = 2; k < 5; ++k) o[k] = acc[k] * s1;
+        o[5] = (double)nreb;
+        for (int k = 5; k < kRecP; ++k) o[k + 1] = acc[k] * s1;
+        o[9] = 0.5 * Q * (xi * xi) + nf * kT * eta;
+        o[10] = 0.0;
+        o[11] = 0.0;
+      }
+      log_count[0] = idx + 1;
+    }
+  }
+  if (start) s2 = half();
+  st[0] = xi;
+  st[1] = eta;
+  st[2] = Q;
+  st[3] = s1;
+  st[4] = s2;
+  st[5] = k2;
+}
+
+// Nose-Hoover half steps of every system (integrators.py:90-112), one block per system:
+// k2 = sum m v^2 is twice the kinetic energy of the brick partials of the fused launch
+// that just closed the step. With `end`, the half step that ends that step (and the
+// record of the step, when `rec`: kinetic energy, momentum and angular momentum scaled
+// as the velocities are, and the bath energy Q xi^2 / 2 + N_f kT eta); with `start`,
+// the half step that opens the next one (k2 of the scaled velocities is k2 scale^2).
+// nh[6 s ..]: xi, eta, Q (0 until the first half step), the velocity scales of the two
+// half steps (1 for a half step not taken; the drift applies them one after the other,
+// as the reference does), and k2 after the last half step (-1 before any; a run's first,
+// start-only launch takes k2 from it, or from the partials of the set-up record).
+__global__ void k_b_nh(const BSys* __restrict__ sys, const double* __restrict__ partials, double* __restrict__ nh,
+                       BInteg ig, int end, int start, int rec, double* __restrict__ log, int log_cap,
+                       int* __restrict__ log_count, const int* __restrict__ nreb) {
+  __shared__ double red[kRecP * (kBT / 64)];
+  const int si = blockIdx.x;
+  const BSys& s = sys[si];
+  double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double* __restrict__ ps = partials + (size_t)s.brick_off * kRecP;
+  for (int b = threadIdx.x; b < s.nbricks; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < kRecP; ++k) acc[k] += ps[(size_t)b * kRecP + k];
+  block_sum<kRecP>(acc, red);
+  if (threadIdx.x != 0) return;
+  nh_update(s, ig, acc, nh + 6 * si, end, start, rec, log + (size_t)si * log_cap * kRecW, log_cap, log_count + si,
+            nreb[si]);
+}
+
 
 // Positions (which = 0), velocities (1) or forces (2) of every slot scattered to user order.
 __global__ void k_b_scatter_user(int natoms, int which, const double4* __restrict__ posA,
@@ -1035,7 +1214,7 @@ struct SysSpec {
   int natoms = 0;
   double box[3] = {0, 0, 0};
   int ns = 1;
-  double mass[kMaxSpecies] = {0}, charge[kMaxSpecies] = {0};
+  double mass[kMaxSpecies] = {0}, charge[kMaxSpecies] = {0}, diff[kMaxSpecies] = {0};
   bool has_lj = false, has_coul = false;
   double eps[64], sig2[64], rc2[64], shift = 0;
   double alpha = 0, rcut = 0, erc = 0, kc = 0;
@@ -1069,6 +1248,9 @@ struct BMD {
   int* flags = nullptr;  // [S] per system + [S] any
   int* ints = nullptr;   // [0] max neighbours, [1] region overflow, [2] rebuild barrier timeout
   int* nreb = nullptr;
+  long long* ndrift = nullptr;  // per system: drifts done (noise counter of the stochastic integrators)
+  double* nh = nullptr;         // per system: Nose-Hoover xi, eta, Q, velocity scales s1, s2, k2
+  BInteg ig = {kVV, 0.0, 0.0, 0.0, 0ull};
   int* bflags = nullptr;   // rebuild flags handed to the list launch
   unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
   int* bsum = nullptr;      // per-block sums of the rebuild scan
@@ -1101,7 +1283,7 @@ struct BMD {
     void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, spar, sta.vel[0],
                   sta.vel[1], sta.F[0], sta.F[1], sta.ref[0], sta.ref[1], sta.perm[0], sta.perm[1], sta.cell[0],
                   sta.cell[1], count, start, tmp, slot, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
-                  bar, bsum, tphase, snap, snap_count, snap_rec};
+                  bar, bsum, tphase, snap, snap_count, snap_rec, ndrift, nh};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1142,6 +1324,7 @@ struct BMD {
     p.flags = flags;
     p.bflags = bflags;
     p.nreb = nreb;
+    p.ndrift = ndrift;
     p.flip_n = flip_n;
     p.rec = rec;
     p.partials = partials;
@@ -1237,10 +1420,45 @@ struct BMD {
   // systems advance one (sub)step with weights wc[t] (closing kick) and wn[t] (next
   // opening kick and drift). The last tick must advance every system. Each system's
   // buffer parity flips with every drift, on the device (inside the rebuild launch).
-  void enqueue(int coarse, int nticks, const int* active, const double* wc, const double* wn, int record_every) {
+  void launch_drift(double wn, int drift) {
     const int T = 256;
     hipLaunchKernelGGL(k_b_drift, dim3((N + T - 1) / T), dim3(T), 0, st, d_sys, d_sysid, N, pos, pos_b, par, sta, spar,
-                       wc[0], flags, S);
+                       wn, flags, S, ig, ndrift, nh, drift);
+  }
+
+  void launch_nh(int end, int start, int rec) {
+    hipLaunchKernelGGL(k_b_nh, dim3(S), dim3(kBT), 0, st, d_sys, partials, nh, ig, end, start, rec, log, log_cap,
+                       log_count, nreb);
+  }
+
+  // The thermostatted integrators (one step per coarse step, every system active): the
+  // fused launch stops after the forces and the closing kick (none for Brownian, whose
+  // velocities are dx/dt), the drift launch does the rest of the step, and Nose-Hoover
+  // adds its half steps in between (which also write its records).
+  void enqueue_thermo(int coarse, int record_every) {
+    const bool nose = ig.mode == kNoseHoover;
+    const double wc = ig.mode == kBrownian ? 0.0 : 1.0;
+    if (nose) launch_nh(0, 1, 0);
+    launch_drift(1.0, 1);
+    launch_rebuild(S);
+    for (int c = 0; c < coarse; ++c) {
+      const bool last = c == coarse - 1;
+      const int rec = record_every > 0 && ((c + 1) % record_every == 0 || last);
+      launch_fused(nbricks, wc, 0.0, 0, nose ? 1 : rec);
+      if (nose) launch_nh(1, !last, rec);
+      else if (rec) launch_record();
+      if (nose) launch_drift(1.0, !last);
+      else if (!last) launch_drift(1.0, 1);
+      if (!last) launch_rebuild(S);
+    }
+  }
+
+  void enqueue(int coarse, int nticks, const int* active, const double* wc, const double* wn, int record_every) {
+    if (ig.mode != kVV) {
+      enqueue_thermo(coarse, record_every);
+      return;
+    }
+    launch_drift(wc[0], 1);
     launch_rebuild(S);
     for (int c = 0; c < coarse; ++c) {
       for (int t = 0; t < nticks; ++t) {
@@ -1292,6 +1510,7 @@ static int build_tables(BMD& m) {
     for (int k = 0; k < sp.ns; ++k) {
       s.mass[k] = sp.mass[k];
       s.charge[k] = sp.charge[k];
+      s.diff[k] = sp.diff[k];
     }
     s.has_lj = sp.has_lj;
     if (sp.has_lj)
@@ -1431,6 +1650,18 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
   HIP_TRY(hipMemset(m->ints, 0, 64));
   HIP_TRY(hipMalloc(&m->nreb, nsys * sizeof(int) + 64));
   HIP_TRY(hipMemset(m->nreb, 0, nsys * sizeof(int) + 64));
+  HIP_TRY(hipMalloc(&m->ndrift, nsys * sizeof(long long)));
+  HIP_TRY(hipMemset(m->ndrift, 0, nsys * sizeof(long long)));
+  {
+    std::vector<double> nh0((size_t)nsys * 6, 0.0);
+    for (int k = 0; k < nsys; ++k) {
+      nh0[6 * k + 3] = 1.0;
+      nh0[6 * k + 4] = 1.0;
+      nh0[6 * k + 5] = -1.0;
+    }
+    HIP_TRY(hipMalloc(&m->nh, nh0.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(m->nh, nh0.data(), nh0.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMalloc(&m->log_count, nsys * sizeof(int) + 64));
   HIP_TRY(hipMemset(m->log_count, 0, nsys * sizeof(int) + 64));
   std::vector<double4> h4(N);
@@ -1619,12 +1850,60 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
 }
 
 // coarse steps of nticks ticks each (see BMD::enqueue). use_graph captures one block.
+int accu_bmd_set_integrator(accu_bmd* h, int mode, double temperature, double friction, double period,
+                            unsigned long long seed) {
+  ARG_CHECK(h, "null handle");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(mode >= kVV && mode <= kNoseHoover, "unknown integrator mode %d", mode);
+  ARG_CHECK(mode == kVV || temperature >= 0, "negative temperature");
+  ARG_CHECK(mode != kBrownian || temperature > 0, "Brownian dynamics needs a positive temperature");
+  ARG_CHECK(mode != kLangevin || friction >= 0, "negative friction");
+  ARG_CHECK(mode != kNoseHoover || (period > 0 && temperature > 0), "Nose-Hoover needs a positive period and temperature");
+  m.ig.mode = mode;
+  m.ig.T = temperature;
+  m.ig.gamma = friction;
+  m.ig.tau = period;
+  m.ig.seed = seed;
+  if (m.gexec) {  // the captured schedule depends on the integrator
+    (void)hipGraphExecDestroy(m.gexec);
+    (void)hipGraphDestroy(m.graph);
+    m.gexec = nullptr;
+    m.graph = nullptr;
+    m.graph_key.clear();
+  }
+  return ACCU_OK;
+}
+
+int accu_bmd_set_diffusion(accu_bmd* h, int sys, int ns, const double* D) {
+  ARG_CHECK(h && D, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(!m.ready, "set diffusion coefficients before accu_bmd_init");
+  ARG_CHECK(sys >= 0 && sys < m.S, "bad system %d", sys);
+  ARG_CHECK(ns == m.spec[sys].ns, "system %d has %d species, not %d", sys, m.spec[sys].ns, ns);
+  for (int k = 0; k < ns; ++k) {
+    ARG_CHECK(D[k] >= 0, "negative diffusion coefficient");
+    m.spec[sys].diff[k] = D[k];
+  }
+  return ACCU_OK;
+}
+
+int accu_bmd_get_thermostat(accu_bmd* h, double* out) {
+  ARG_CHECK(h && out, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  HIP_TRY(hipMemcpyAsync(out, m.nh, (size_t)m.S * 6 * sizeof(double), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  return ACCU_OK;
+}
+
 int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const double* wc, const double* wn,
                  int record_every, int use_graph) {
   ARG_CHECK(h && active && wc && wn, "null argument");
   BMD& m = *reinterpret_cast<BMD*>(h);
   ARG_CHECK(m.ready, "call accu_bmd_init first");
   ARG_CHECK(coarse >= 1 && nticks >= 1 && nticks <= 16, "bad step counts");
+  ARG_CHECK(m.ig.mode == kVV || (nticks == 1 && wc[0] == 1.0 && wn[0] == 1.0),
+            "the thermostatted integrators advance every system one step of dt per coarse step");
+  ARG_CHECK(!(m.ig.mode == kNoseHoover && m.snap), "snapshot frames are not taken with the Nose-Hoover integrator");
   for (int t = 0; t < nticks; ++t) ARG_CHECK(active[t] >= 1 && active[t] <= m.S, "bad active count at tick %d", t);
   ARG_CHECK(active[nticks - 1] == m.S, "the last tick must advance every system");
   std::vector<int> cnt(m.S);

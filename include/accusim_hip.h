@@ -166,6 +166,21 @@ int accu_bmd_get_snapshots(accu_bmd* h, int first, int count, double* pos_host, 
                            int* taken_host);
 int accu_bmd_set_timing(accu_bmd* h, int enable);
 int accu_bmd_timing(accu_bmd* h, double* total_ms_host, long long* launches_host);
+/* integrator of every system of the arena: mode 0 velocity Verlet / Yoshida4 (the
+ * accu_bmd_run weights), 1 Langevin BAOAB (integrators.py:50-71: temperature, friction
+ * gamma), 2 Brownian (integrators.py:237-257: temperature, each species' diffusion from
+ * accu_bmd_set_diffusion), 3 Nose-Hoover (integrators.py:74-116: temperature, period);
+ * modes 1-3 run one step of dt per coarse step (accu_bmd_run with one tick, weights 1).
+ * Langevin and Brownian noise: Philox4x32-10 normals keyed by seed, one counter per drift
+ * of a system, one stream per atom user index. Nose-Hoover records carry the bath energy
+ * (Q xi^2 / 2 + N_f kT eta, integrators.py:114-116) in log column 9. */
+int accu_bmd_set_integrator(accu_bmd* h, int mode, double temperature, double friction, double period,
+                            unsigned long long seed);
+/* per-species diffusion coefficients D [ns] of system sys (Species.diffusion, system.py:17), before accu_bmd_init */
+int accu_bmd_set_diffusion(accu_bmd* h, int sys, int ns, const double* D_host);
+/* Nose-Hoover state per system to out_host [nsys][6]: xi, eta, Q, the velocity scales of the
+ * last two half steps, k2 */
+int accu_bmd_get_thermostat(accu_bmd* h, double* out_host);
 
 /* ---- learned DSF -> Ewald pair correction (no counterpart in the reference; it
  * corrects DampedShiftedCoulomb, forces.py:391-452, toward Ewald, forces.py:333-388).

@@ -63,6 +63,7 @@ class Species:
     name: str
     mass: float
     charge: float = 0.0
+    diffusion: float = 0.0
 
 
 class System:
@@ -81,11 +82,11 @@ class System:
         self.ids = np.zeros(0, dtype=int)
         self._next_id = 0
 
-    def add_species(self, name, mass, charge=0.0):
+    def add_species(self, name, mass, charge=0.0, diffusion=0.0):
         # system.py:42-50
         if name in self._index:
             raise ValueError(f"species '{name}' already defined")
-        s = Species(name, float(mass), float(charge))
+        s = Species(name, float(mass), float(charge), float(diffusion))
         if s.mass <= 0:
             raise ValueError("mass must be positive")
         self._index[name] = len(self.species)
@@ -733,6 +734,174 @@ class Yoshida4:
             self._vv.step(sim, w * dt)
 
 
+class Langevin:
+    """integrators.py:50-71: BAOAB Langevin dynamics (kick, half drift, Ornstein-Uhlenbeck
+    velocity update, half drift, forces, kick)."""
+    order = 2
+    symplectic = False
+    stochastic = True
+    thermostat = True
+
+    def __init__(self, temperature, friction):
+        self.T = float(temperature)
+        self.gamma = float(friction)
+
+    def step(self, sim, dt):
+        s = sim.system
+        s.vel += 0.5 * dt * sim.acc
+        s.pos += 0.5 * dt * s.vel
+        c1 = np.exp(-self.gamma * dt)
+        sig = np.sqrt((1 - c1 ** 2) * K_B * self.T / s.masses)[:, None]
+        s.vel = c1 * s.vel + sig * sim.rng.normal(size=s.vel.shape)
+        s.pos += 0.5 * dt * s.vel
+        s.apply_boundaries()
+        sim.update_forces()
+        s.vel += 0.5 * dt * sim.acc
+
+
+class NoseHoover:
+    """integrators.py:74-116: Nose-Hoover chain of length one in the Trotter splitting of
+    Martyna, Tuckerman and Klein (1996); thermostat mass Q = N_f kT tau^2 at the first
+    half step, N_f = dim (n - 1); conserved extended energy E + Q xi^2 / 2 + N_f kT eta."""
+    order = 2
+    symplectic = False
+    stochastic = False
+    thermostat = True
+
+    def __init__(self, temperature, period):
+        self.T = float(temperature)
+        self.tau = float(period)
+        self.xi = 0.0
+        self.eta = 0.0
+        self.Q = None
+
+    @staticmethod
+    def dof(s):
+        return s.dim * max(s.n - 1, 1)
+
+    def _half(self, s, dt):
+        kT, nf = K_B * self.T, self.dof(s)
+        if self.Q is None:
+            self.Q = nf * kT * self.tau ** 2
+        k2 = float(np.sum(s.masses[:, None] * s.vel ** 2))
+        self.xi += 0.25 * dt * (k2 - nf * kT) / self.Q
+        scale = np.exp(-0.5 * dt * self.xi)
+        s.vel *= scale
+        self.eta += 0.5 * dt * self.xi
+        self.xi += 0.25 * dt * (k2 * scale ** 2 - nf * kT) / self.Q
+
+    def step(self, sim, dt):
+        s = sim.system
+        self._half(s, dt)
+        s.vel += 0.5 * dt * sim.acc
+        s.pos += dt * s.vel
+        s.apply_boundaries()
+        sim.update_forces()
+        s.vel += 0.5 * dt * sim.acc
+        self._half(s, dt)
+
+    def bath_energy(self, s):
+        if self.Q is None:
+            return 0.0
+        return 0.5 * self.Q * self.xi ** 2 + self.dof(s) * K_B * self.T * self.eta
+
+
+class Brownian:
+    """integrators.py:237-257: overdamped Langevin (Euler-Maruyama) with each species'
+    diffusion coefficient D: dx = D F / (kT) dt + sqrt(2 D dt) xi, v = dx / dt."""
+    order = 1
+    symplectic = False
+    stochastic = True
+    thermostat = True
+
+    def __init__(self, temperature):
+        self.T = float(temperature)
+
+    def step(self, sim, dt):
+        s = sim.system
+        D = np.array([sp.diffusion for sp in s.species])[s.sp][:, None]
+        drift = D * sim.force / (K_B * self.T) * dt
+        noise = np.sqrt(2 * D * dt) * sim.rng.normal(size=s.pos.shape)
+        s.pos += drift + noise
+        s.vel = (drift + noise) / dt
+        s.apply_boundaries()
+        sim.update_forces()
+
+
+# ---------------------------------------------------------------- counter-based normals
+# The device engines draw their Gaussian noise from Philox4x32-10 (Salmon, Moraes, Dror
+# and Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC11) instead of numpy's
+# PCG64 stream. This is a restatement of that generator and of the engines' mapping to
+# normals, so that the reference's stochastic integrators above can be driven with the
+# same noise (as `sim.rng`) and compared with the device step for step. Pinned by the
+# known-answer vectors of the Random123 distribution (tests/test_oracle_thermostats.py).
+_PH_M0, _PH_M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_PH_W0, _PH_W1 = np.uint32(0x9E3779B9), np.uint32(0xBB67AE85)
+
+
+def philox4x32_10(ctr, key):
+    """ctr: uint32 [..., 4], key: uint32 [..., 2] -> uint32 [..., 4]."""
+    c = [np.asarray(ctr[..., i], np.uint32).copy() for i in range(4)]
+    k0 = np.asarray(key[..., 0], np.uint32).copy()
+    k1 = np.asarray(key[..., 1], np.uint32).copy()
+    mask = np.uint64(0xFFFFFFFF)
+    for r in range(10):
+        if r:
+            k0 = (k0 + _PH_W0).astype(np.uint32)
+            k1 = (k1 + _PH_W1).astype(np.uint32)
+        p0 = _PH_M0 * c[0].astype(np.uint64)
+        p1 = _PH_M1 * c[2].astype(np.uint64)
+        hi0, lo0 = (p0 >> np.uint64(32)).astype(np.uint32), (p0 & mask).astype(np.uint32)
+        hi1, lo1 = (p1 >> np.uint64(32)).astype(np.uint32), (p1 & mask).astype(np.uint32)
+        c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+    return np.stack(c, axis=-1)
+
+
+def philox_normals(seed, counter, ids):
+    """Three standard normals per id for drift number `counter` (the engines' noise):
+    Philox4x32-10 with key (seed low, seed high) at counters (counter low, counter high,
+    id, j) for j = 0, 1 gives 8 words; words (2i, 2i+1) make the uniform
+    ((w0 >> 5) 2^26 + (w1 >> 6) + 0.5) 2^-53 in (0, 1); uniforms (u0, u1) and (u2, u3)
+    go through Box-Muller, r = sqrt(-2 ln u0), n = (r cos 2 pi u1, r sin 2 pi u1, ...)."""
+    ids = np.asarray(ids, np.uint64)
+    n = len(ids)
+    key = np.zeros((n, 2), np.uint32)
+    key[:, 0] = np.uint32(seed & 0xFFFFFFFF)
+    key[:, 1] = np.uint32((seed >> 32) & 0xFFFFFFFF)
+    words = []
+    for j in range(2):
+        ctr = np.zeros((n, 4), np.uint32)
+        ctr[:, 0] = np.uint32(counter & 0xFFFFFFFF)
+        ctr[:, 1] = np.uint32((counter >> 32) & 0xFFFFFFFF)
+        ctr[:, 2] = ids.astype(np.uint32)
+        ctr[:, 3] = np.uint32(j)
+        words.append(philox4x32_10(ctr, key))
+    w = np.concatenate(words, axis=1).astype(np.float64)  # [n, 8]
+    u = ((np.floor(w[:, 0::2] / 32.0)) * 67108864.0 + np.floor(w[:, 1::2] / 64.0) + 0.5) * 2.0 ** -53
+    r0 = np.sqrt(-2.0 * np.log(u[:, 0]))
+    r1 = np.sqrt(-2.0 * np.log(u[:, 2]))
+    t0 = 2.0 * np.pi * u[:, 1]
+    t1 = 2.0 * np.pi * u[:, 3]
+    return np.stack([r0 * np.cos(t0), r0 * np.sin(t0), r1 * np.cos(t1)], axis=1)
+
+
+class PhiloxNormals:
+    """Drop-in for `sim.rng` in the stochastic integrators: each normal() call is one
+    drift, numbered from 0, over the atoms of the system in order (ids 0..n-1, offset by
+    `id0` for a system placed after others in a device arena)."""
+
+    def __init__(self, seed, id0=0):
+        self.seed = int(seed)
+        self.id0 = int(id0)
+        self.counter = 0
+
+    def normal(self, size):
+        n = size[0]
+        out = philox_normals(self.seed, self.counter, np.arange(self.id0, self.id0 + n))
+        self.counter += 1
+        return out[:, :size[1]] if len(size) > 1 else out[:, 0]
+
+
 @dataclass
 class Trajectory:
     """simulation.py:14-38 (fields on the force path)."""
@@ -743,6 +912,7 @@ class Trajectory:
     angular_momentum: list = field(default_factory=list)
     mass: list = field(default_factory=list)
     charge: list = field(default_factory=list)
+    bath: list = field(default_factory=list)
     positions: list = field(default_factory=list)
     velocities: list = field(default_factory=list)
 
@@ -760,6 +930,7 @@ class Simulation:
         self.forces = list(forces)
         self.integrator = integrator or VelocityVerlet()
         self.dt = dt
+        self.rng = np.random.default_rng(seed)
         self.seed = seed
         self.time = 0.0
         self.steps = 0
@@ -794,6 +965,8 @@ class Simulation:
         tr.angular_momentum.append(s.angular_momentum() if s.dim in (2, 3) else np.zeros(1))
         tr.mass.append(float(np.sum(s.masses)))
         tr.charge.append(float(np.sum(s.charges)))
+        bath = getattr(self.integrator, "bath_energy", None)
+        tr.bath.append(bath(s) if bath else 0.0)
         if positions:
             tr.positions.append(s.pos.copy())
             tr.velocities.append(s.vel.copy())
