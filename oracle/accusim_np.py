@@ -33,6 +33,15 @@ How each part is pinned (tests in tests/test_oracle_*.py):
                     rotor (uniform rotation at the exact angular velocity)
   record / report   drift formulas on a trajectory with a known answer,
                     golden vectors of a short argon run
+  Langevin          the reference's thermostat test (tests/test_dynamics.py:80-95:
+                    50 K -> 200 K within 8 %), the closed form at T = 0 (velocity
+                    times c1 per step)
+  NoseHoover        the reference's test (tests/test_nose_hoover.py:20-36): target
+                    temperature within 5 %, extended energy kept to 1e-3, its error
+                    shrinking like dt^2
+  Brownian          free diffusion: positions equal the summed noise, mean square
+                    displacement 6 D t
+  philox4x32_10     the Random123 known-answer vectors; the normals' moments
 """
 from __future__ import annotations
 
