@@ -2104,6 +2104,19 @@ int accu_bmd_get_snapshots(accu_bmd* h, int first, int count, double* pos_host, 
   return ACCU_OK;
 }
 
+int accu_bmd_snapshot_frames(accu_bmd* h, void** frames_out, int* stored_out) {
+  ARG_CHECK(h && frames_out && stored_out, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  int taken = 0;
+  if (m.snap_count) {
+    HIP_TRY(hipMemcpyAsync(&taken, m.snap_count, sizeof(int), hipMemcpyDeviceToHost, m.st));
+    HIP_TRY(hipStreamSynchronize(m.st));
+  }
+  *frames_out = m.snap;
+  *stored_out = std::min(taken, m.snap_cap);
+  return ACCU_OK;
+}
+
 int accu_bmd_set_timing(accu_bmd* h, int enable) {
   ARG_CHECK(h, "null handle");
   BMD& m = *reinterpret_cast<BMD*>(h);

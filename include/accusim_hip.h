@@ -164,6 +164,10 @@ int accu_bmd_set_snapshots(accu_bmd* h, int cap_frames, int every);
  * taken so far (including those beyond the capacity) */
 int accu_bmd_get_snapshots(accu_bmd* h, int first, int count, double* pos_host, double* vel_host, int* rec_host,
                            int* taken_host);
+/* the snapshot ring in device memory, for the analysis entry points: *frames_out = frame 0
+ * (each frame N positions then N velocities [N][3] in user order, 6 N doubles), *stored_out
+ * = frames stored; valid until the ring is resized or the engine destroyed */
+int accu_bmd_snapshot_frames(accu_bmd* h, void** frames_out, int* stored_out);
 int accu_bmd_set_timing(accu_bmd* h, int enable);
 int accu_bmd_timing(accu_bmd* h, double* total_ms_host, long long* launches_host);
 /* integrator of every system of the arena: mode 0 velocity Verlet / Yoshida4 (the
@@ -181,6 +185,22 @@ int accu_bmd_set_diffusion(accu_bmd* h, int sys, int ns, const double* D_host);
 /* Nose-Hoover state per system to out_host [nsys][6]: xi, eta, Q, the velocity scales of the
  * last two half steps, k2 */
 int accu_bmd_get_thermostat(accu_bmd* h, double* out_host);
+
+/* ---- analysis (accusim/analysis.py). pos / frames are device float64 arrays.
+ * rdf pair counts (analysis.py:7-24): every pair i < j of pos [n][3] (minimum image when
+ * periodic, box_host [3]) with |d| <= rmax, binned as numpy.histogram(r, bins, (0, rmax))
+ * bins it, edges_host [bins + 1] being numpy.linspace(0, rmax, bins + 1); counts to
+ * counts_host [bins] (bins <= 2048). */
+int accu_rdf_counts(const double* pos, int n, const double* box_host, int periodic, double rmax,
+                    const double* edges_host, int bins, unsigned long long* counts_host, void* stream);
+/* msd (analysis.py:27-43) over nframes frames of n atoms, frame f at frames + f * stride
+ * (doubles; [n][3] each), minimum-image steps when periodic; out_host [nframes] */
+int accu_msd(const double* frames, int nframes, long long stride, int n, const double* box_host, int periodic,
+             double* out_host, void* stream);
+/* vacf (analysis.py:77-87): out_host [max_lag] = mean over atoms and origins t < nframes - k
+ * of v(t) . v(t + k), velocity frames laid out as in accu_msd */
+int accu_vacf(const double* frames, int nframes, long long stride, int n, int max_lag, double* out_host,
+              void* stream);
 
 /* ---- learned DSF -> Ewald pair correction (no counterpart in the reference; it
  * corrects DampedShiftedCoulomb, forces.py:391-452, toward Ewald, forces.py:333-388).

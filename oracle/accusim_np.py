@@ -42,6 +42,13 @@ How each part is pinned (tests in tests/test_oracle_*.py):
   Brownian          free diffusion: positions equal the summed noise, mean square
                     displacement 6 D t
   philox4x32_10     the Random123 known-answer vectors; the normals' moments
+  rdf               pair counts of a perfect FCC crystal (12 N / 2 nearest-neighbour
+                    pairs, 6 N / 2 at the next shell), brute-force pair loop on a
+                    random box, g -> 1 for an ideal gas
+  msd / vacf        ballistic closed forms (msd = <v^2> t^2, C = <v^2>), unwrapping
+                    across box faces; green_kubo_diffusion = kT / (m gamma) for
+                    free Langevin particles (tests/test_analysis.py:22-35)
+  pressure          ideal gas P = N kT / V and the virial of LJ pairs
 """
 from __future__ import annotations
 
@@ -114,6 +121,10 @@ class System:
         self.sp = np.concatenate([self.sp, np.full(n, self._index[species])])
         self.ids = np.concatenate([self.ids, new_ids])
         return new_ids
+
+    def species_index(self, name):
+        # system.py:52-53
+        return self._index[name]
 
     @property
     def n(self):
@@ -1059,6 +1070,85 @@ def force_approx_check(force, s):
     F_ex = direct_subset(force, s, sub)
     e = rms_rel_force_error(F_ap, F_ex)
     return _grade(e, 1e-3, 1e-2), e
+
+
+# ---------------------------------------------------------------- analysis.py:7-94
+
+def rdf(system, rmax, bins=100, species=None):
+    """analysis.py:7-24."""
+    r, g, _ = rdf_counts(system, rmax, bins, species)
+    return r, g
+
+
+def rdf_counts(system, rmax, bins=100, species=None):
+    """analysis.py:7-24, also returning the pair histogram."""
+    s = system
+    idx = np.arange(s.n) if species is None else np.flatnonzero(s.sp == s.species_index(species))
+    if not s.periodic:
+        raise ValueError("rdf needs a periodic box (for normalisation)")
+    if rmax > 0.5 * np.min(np.broadcast_to(s.box, (s.dim,))):
+        raise ValueError("rmax must be at most half the box")
+    p = s.pos[idx]
+    d = p[None, :, :] - p[:, None, :]
+    d -= s.box * np.round(d / s.box)
+    r = np.sqrt(np.einsum("ijk,ijk->ij", d, d))[np.triu_indices(len(idx), 1)]
+    hist, edges = np.histogram(r, bins=bins, range=(0, rmax))
+    V = float(np.prod(np.broadcast_to(s.box, (s.dim,))))
+    n = len(idx)
+    shell = (4 / 3 * np.pi * (edges[1:] ** 3 - edges[:-1] ** 3)) if s.dim == 3 else (np.pi * (edges[1:] ** 2 - edges[:-1] ** 2))
+    ideal = shell * n * (n - 1) / 2 / V
+    return 0.5 * (edges[1:] + edges[:-1]), hist / ideal, hist
+
+
+def msd(trajectory, box=None):
+    """analysis.py:27-43."""
+    t = np.asarray(trajectory.t)
+    frames = trajectory.positions
+    disp = np.zeros_like(frames[0])
+    out = [0.0]
+    for a, b in zip(frames[:-1], frames[1:]):
+        step = b - a
+        if box is not None:
+            step -= box * np.round(step / box)
+        disp = disp + step
+        out.append(float(np.mean(np.sum(disp ** 2, axis=1))))
+    return t - t[0], np.array(out)
+
+
+def pressure(sim, h=1e-6):
+    """analysis.py:46-64."""
+    s = sim.system
+    if s.box is None:
+        raise ValueError("pressure needs a box")
+    box0, pos0 = np.array(s.box, dtype=float), s.pos.copy()
+    V0 = float(np.prod(np.broadcast_to(box0, (s.dim,))))
+    U = []
+    for sign in (+1, -1):
+        lam = (1 + sign * h) ** (1 / s.dim)
+        s.box, s.pos = box0 * lam, pos0 * lam
+        U.append(sum(f.compute(s)[1] for f in sim.forces if not f.external))
+    s.box, s.pos = (box0 if np.ndim(box0) else float(box0)), pos0
+    dUdV = (U[0] - U[1]) / (2 * h * V0)
+    return 2 * s.kinetic_energy() / (s.dim * V0) - dUdV
+
+
+def vacf(trajectory, max_lag=None):
+    """analysis.py:77-87."""
+    V = np.asarray(trajectory.velocities)
+    if V.ndim != 3:
+        raise ValueError("vacf needs recorded velocities of a constant particle set")
+    n = len(V)
+    max_lag = n // 2 if max_lag is None else max_lag
+    C = np.array([np.mean(np.einsum("tij,tij->ti", V[:n - k], V[k:])) for k in range(max_lag)])
+    t = np.asarray(trajectory.t)
+    return t[:max_lag] - t[0], C
+
+
+def green_kubo_diffusion(trajectory, max_lag=None):
+    """analysis.py:90-94."""
+    t, C = vacf(trajectory, max_lag)
+    dim = np.asarray(trajectory.velocities).shape[2]
+    return float(np.sum(0.5 * (C[1:] + C[:-1]) * np.diff(t))) / dim
 
 
 def timestep_convergence(sim, duration, start=None):
