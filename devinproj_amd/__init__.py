@@ -9,7 +9,7 @@ from .batch import BrickEngine
 from .engine import MDEngine
 from .forces import (DampedShiftedCoulomb, Ewald, Force, Gravity, LennardJones, NeighbourList, VerletList,
                      cell_list_pairs, kvector_columns)
-from .integrators import Brownian, Integrator, Langevin, NoseHoover, VelocityVerlet, Yoshida4
+from .integrators import Brownian, Integrator, Langevin, NoseHoover, Rattle, VelocityVerlet, Yoshida4
 from . import autotune, batch, workloads
 from .learned import LearnedDSF, PairCorrection, train_correction
 from .simulation import Simulation, Trajectory
@@ -20,7 +20,7 @@ from .units import G, k_B, k_e, si
 __all__ = [
     "AccuracyReport", "BarnesHut", "BrickEngine", "Brownian", "Check", "DampedShiftedCoulomb", "Ewald", "Force", "G",
     "Gravity", "Integrator", "Langevin", "LearnedDSF", "LennardJones", "MDEngine", "NeighbourList", "NoseHoover",
-    "PairCorrection", "Simulation", "Species", "System",
+    "PairCorrection", "Rattle", "Simulation", "Species", "System",
     "Trajectory", "VelocityVerlet", "train_correction",
     "VerletList", "Yoshida4", "cell_list_pairs", "direct_subset", "k_B", "k_e", "kvector_columns", "si",
     "timestep_convergence",

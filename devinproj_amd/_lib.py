@@ -182,6 +182,8 @@ SIGNATURES.update({
     "accu_bmd_set_integrator": [P, I, D, D, D, C.c_ulonglong],
     "accu_bmd_set_diffusion": [P, I, I, P],
     "accu_bmd_get_thermostat": [P, P],
+    "accu_bmd_set_constraints": [P, I, P, P, P, P, P, P, P, D, I],
+    "accu_bmd_get_constraints": [P, P, P],
     "accu_bmd_snapshot_frames": [P, P, P],
     "accu_rdf_counts": [P, I, P, I, D, P, I, P, P],
     "accu_msd": [P, I, LL, I, P, I, P, P],

@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib as L
 from .accuracy import richardson_check
 from .forces import DampedShiftedCoulomb, LennardJones, _erfc
-from .integrators import Brownian, Langevin, NoseHoover, VelocityVerlet, Yoshida4
+from .integrators import Brownian, Langevin, NoseHoover, Rattle, VelocityVerlet, Yoshida4, constraint_clusters
 from .simulation import Trajectory
 
 LOG_W = 12  # PE, KE, Px, Py, Pz, rebuilds, Lx, Ly, Lz, 0, 0, 0
@@ -76,9 +76,11 @@ class BrickEngine:
         elif isinstance(ig, VelocityVerlet):
             self.ticks = (1, [S], [1.0], [1.0])
             self._mode = (0, 0.0, 0.0, 0.0, 0)
-        elif isinstance(ig, (Langevin, NoseHoover, Brownian)):
+        elif isinstance(ig, (Langevin, NoseHoover, Brownian, Rattle)):
             self.ticks = (1, [S], [1.0], [1.0])
-            if isinstance(ig, Langevin):
+            if isinstance(ig, Rattle):
+                self._mode = (4, 0.0, 0.0, 0.0, 0)
+            elif isinstance(ig, Langevin):
                 self._mode = (1, ig.T, ig.gamma, 0.0, ig.seed)
             elif isinstance(ig, Brownian):
                 self._mode = (2, ig.T, 0.0, 0.0, ig.seed)
@@ -87,7 +89,7 @@ class BrickEngine:
                     raise ValueError("the device Nose-Hoover starts from a fresh thermostat (xi = eta = 0)")
                 self._mode = (3, ig.T, 0.0, ig.tau, 0)
         else:
-            raise ValueError("the brick engine runs VelocityVerlet, Yoshida4, Langevin, NoseHoover or Brownian")
+            raise ValueError("the brick engine runs VelocityVerlet, Yoshida4, Langevin, NoseHoover, Brownian or Rattle")
         self.precision = precision
         L.lib()
         for s in self.systems:
@@ -137,6 +139,8 @@ class BrickEngine:
             if self._mode[0] == 2:
                 D = np.ascontiguousarray([x.diffusion for x in s.species], float)
                 L.call("accu_bmd_set_diffusion", h, k, len(s.species), L.ptr(D))
+        if self._mode[0] == 4:
+            self._set_constraints(ig)
         mode, T, gamma, tau, seed = self._mode
         L.call("accu_bmd_set_integrator", h, mode, float(T), float(gamma), float(tau),
                L.C.c_ulonglong(int(seed) & 0xFFFFFFFFFFFFFFFF))
@@ -151,6 +155,59 @@ class BrickEngine:
         # simulated time of every record, per system (the set-up record is at t = 0)
         self._clock = np.zeros(S)
         self._rec_times = [self._clock.copy()]
+
+    def _set_constraints(self, rat):
+        """Rattle's bonds as connected clusters per system (accu_bmd_set_constraints): the
+        bonds whose two atoms are in a system apply to it (ids are per system, as in the
+        reference's Rattle.step), each system counts them as constrained degrees of freedom."""
+        coff, aoff, la, lb, ln, uatom, csys = [0], [0], [], [], [], [], []
+        off = 0
+        for k, s in enumerate(self.systems):
+            where = {int(pid): n for n, pid in enumerate(s.ids)}
+            cons = [(where[a], where[b], d) for a, b, d in rat.constraints if a in where and b in where]
+            s.constrained_dof = len(cons)
+            ia = [c[0] for c in cons]
+            ib = [c[1] for c in cons]
+            for g in constraint_clusters(ia, ib):
+                atoms = []
+                loc = {}
+                for q in g:
+                    for x in (ia[q], ib[q]):
+                        if x not in loc:
+                            loc[x] = len(atoms)
+                            atoms.append(x)
+                    la.append(loc[ia[q]])
+                    lb.append(loc[ib[q]])
+                    ln.append(cons[q][2])
+                uatom.extend(off + x for x in atoms)
+                coff.append(len(la))
+                aoff.append(len(uatom))
+                csys.append(k)
+            off += s.n
+        if not la:
+            raise ValueError("Rattle: no constraint has both atoms in a system")
+        arr = [np.ascontiguousarray(x, np.int32) for x in (coff, aoff, la, lb)]
+        lens = np.ascontiguousarray(ln, float)
+        ua, cs = np.ascontiguousarray(uatom, np.int32), np.ascontiguousarray(csys, np.int32)
+        L.call("accu_bmd_set_constraints", self._h, len(csys), *[L.ptr(a) for a in arr], L.ptr(lens), L.ptr(ua),
+               L.ptr(cs), float(rat.tol), int(rat.max_iter))
+
+    def constraint_state(self):
+        """(worst relative bond-length error after any SHAKE so far, error code, cluster,
+        residual at the failure) of a Rattle engine; raises the reference's RuntimeError
+        when a SHAKE or RATTLE solve failed."""
+        out = np.zeros(2)
+        err = np.zeros(2, np.int32)
+        L.call("accu_bmd_get_constraints", self._h, L.ptr(out), L.ptr(err))
+        code = int(err[0])
+        if code == 1:
+            raise RuntimeError("SHAKE did not converge: reduce dt")
+        if code == 2:
+            raise RuntimeError(f"SHAKE stalled at relative residual {out[1]:.1e} > tol {self.integrator.tol:.0e}: "
+                               "raise tol (float64 limit ~ eps * coordinate size / bond length) or reduce dt")
+        if code == 3:
+            raise RuntimeError(f"RATTLE: singular velocity system in constraint cluster {int(err[1])}")
+        return float(out[0])
 
     def run(self, steps, record_every=1, use_graph=False, sync=True, schedule=None):
         nt, active, wc, wn = schedule or self.ticks
@@ -195,6 +252,8 @@ class BrickEngine:
                                           for k, (n, e) in enumerate(zip(names, ends))}}
         self.rebuilds = stats[:, 0].copy()
         self.logs = [log[k, :int(stats[k, 1])].copy() for k in range(S)]
+        if isinstance(self.integrator, Rattle):
+            self.integrator.max_violation = max(self.integrator.max_violation, self.constraint_state())
         if state:
             off = 0
             self.forces_out = []

@@ -266,6 +266,7 @@ struct RB {
   int* bflags;  // the same, as handed to the list launch
   int* nreb;
   long long* ndrift;  // per system: drifts done (the stochastic integrators' noise counter)
+  int* islot;         // slot of every user index (rigid bonds only, else null)
   int S, N, ncells, nbricks;
   int flip_n;    // systems [0, flip_n) drifted: their parity flips with this launch
   int bin_only;  // 1: stop after the scan (set-up measures the regions)
@@ -573,8 +574,10 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       (pc ? p.posA : p.posB)[t] = x;
       p.st.ref[sc ^ 1][t] = x;
       p.st.vel[sc ^ 1][t] = p.st.vel[sc][v];
-      p.st.perm[sc ^ 1][t] = p.st.perm[sc][v];
+      const int u = p.st.perm[sc][v];
+      p.st.perm[sc ^ 1][t] = u;
       p.st.cell[sc ^ 1][t] = c;
+      if (p.islot) p.islot[u] = t;
     };
     for (int cp = 2 * gw; cp < p.ncells; cp += 2 * nw) {
       const int cm = cp + half;  // this half-wave's cell
@@ -998,7 +1001,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
 }
 
 // Integrator of the engine (every system of an arena uses the same one).
-enum { kVV = 0, kLangevin = 1, kBrownian = 2, kNoseHoover = 3 };
+enum { kVV = 0, kLangevin = 1, kBrownian = 2, kNoseHoover = 3, kRattle = 4 };
 struct BInteg {
   int mode;
   double gamma;  // Langevin friction (1/s)
@@ -1230,6 +1233,298 @@ __global__ void k_b_scatter_user(int natoms, int which, const double4* __restric
   out[3 * (size_t)u + 2] = v.z;
 }
 
+// ------------------------------------------------------------------ rigid bonds
+
+constexpr int kMaxClusterBonds = 512;  // bonds of one cluster (its solve is one thread's)
+
+// Constraint clusters of the Rattle integrator (integrators.py:119-234): connected
+// components of the bond graph, one thread each. Bonds couple only through shared atoms,
+// so the reference's dense Newton system is block diagonal over these clusters. Cluster c
+// has bonds [coff[c], coff[c+1]) with local atom indices la / lb and lengths len, atoms
+// [aoff[c], aoff[c+1]) with global user indices uatom; clusters are ordered by system
+// (those of system s are [csoff[s], csoff[s+1])). islot[u] is the current slot of user
+// index u, written by every list rebuild. Work memory per cluster: 8 doubles per bond
+// (old bond vector, drifted bond vector, summed multiplier), 4 per atom (displacement,
+// inverse mass), K^2 + K at moff[c] (the Newton matrix and its right-hand side).
+struct BCons {
+  int ncl;
+  const int *coff, *aoff, *la, *lb, *uatom, *csys, *csoff;
+  const double* len;
+  const long long* moff;
+  int* islot;
+  double *work, *disp, *mat, *cdelta;
+  int* cerr;                     // [0] 1: no convergence in max_iter, 2: stalled; [1] cluster
+  double* cres;                  // [0] residual at the failure
+  unsigned long long* maxviol;   // bits of the worst relative bond-length error (non-negative double)
+  double tol;
+  int max_iter;
+};
+
+__device__ __forceinline__ double mi_np(double d, double L) { return d - L * rint(d / L); }
+
+// A x = b for a K x K row-major A (overwritten) by Gaussian elimination with partial
+// pivoting; x replaces b. False for a singular A.
+__device__ bool gauss_solve(double* __restrict__ A, double* __restrict__ b, int K) {
+  for (int j = 0; j < K; ++j) {
+    int p = j;
+    double best = fabs(A[j * K + j]);
+    for (int i = j + 1; i < K; ++i)
+      if (fabs(A[i * K + j]) > best) {
+        best = fabs(A[i * K + j]);
+        p = i;
+      }
+    if (best == 0.0) return false;
+    if (p != j) {
+      for (int l = j; l < K; ++l) {
+        const double t = A[j * K + l];
+        A[j * K + l] = A[p * K + l];
+        A[p * K + l] = t;
+      }
+      const double t = b[j];
+      b[j] = b[p];
+      b[p] = t;
+    }
+    const double piv = A[j * K + j];
+    for (int i = j + 1; i < K; ++i) {
+      const double f = A[i * K + j] / piv;
+      if (f == 0.0) continue;
+      for (int l = j + 1; l < K; ++l) A[i * K + l] -= f * A[j * K + l];
+      b[i] -= f * b[j];
+    }
+  }
+  for (int j = K - 1; j >= 0; --j) {
+    double s = b[j];
+    for (int l = j + 1; l < K; ++l) s -= A[j * K + l] * b[l];
+    b[j] = s / A[j * K + j];
+  }
+  return true;
+}
+
+// C[k][l] of a cluster: the change of bond vector k per unit multiplier on bond l, which
+// moves atom la[l] by +inv and lb[l] by -inv along that bond (integrators.py:202-205).
+__device__ __forceinline__ double coupling_kl(const int* la, const int* lb, const double* D, int k, int l) {
+  const int ak = la[k], bk = lb[k], al = la[l], bl = lb[l];
+  const double ia = D[4 * ak + 3], ib = D[4 * bk + 3];
+  return ia * ((ak == al ? 1.0 : 0.0) - (ak == bl ? 1.0 : 0.0)) - ib * ((bk == al ? 1.0 : 0.0) - (bk == bl ? 1.0 : 0.0));
+}
+
+// D[j] = inv_j sum_l s(j, l) G_l ro_l: the displacement of every atom of the cluster for
+// summed multipliers G (s = +1 for the bond's first atom, -1 for its second).
+__device__ void cluster_disp(const int* la, const int* lb, const double* W, double* D, int K, int na) {
+  for (int j = 0; j < na; ++j) D[4 * j] = D[4 * j + 1] = D[4 * j + 2] = 0.0;
+  for (int l = 0; l < K; ++l) {
+    const double* w = W + 8 * l;
+    const int a = la[l], b = lb[l];
+    const double ga = w[6] * D[4 * a + 3], gb = w[6] * D[4 * b + 3];
+    for (int d = 0; d < 3; ++d) {
+      D[4 * a + d] += ga * w[d];
+      D[4 * b + d] -= gb * w[d];
+    }
+  }
+}
+
+// SHAKE of every cluster after the drift (integrators.py:197-226): the drifted positions
+// (the next buffer, not yet committed) move along the old bond vectors until every bond's
+// relative residual |L^2 - r^2| / L^2 is at most tol, by Newton iterations on the summed
+// multipliers G: A g = L^2 - r^2 with A[k][l] = 2 C[k][l] (r_k . ro_l). Positions and
+// velocities take the total displacement (velocities / dt), positions are wrapped again,
+// moved atoms are checked against the list's reference positions, and the worst relative
+// bond-length error is kept. A failure leaves the cluster as drifted and sets cerr.
+__global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ posA, double4* __restrict__ posB,
+                          const int* __restrict__ par, BState sta, const int* __restrict__ spar, BCons c,
+                          int* __restrict__ flags, int S) {
+#pragma clang fp contract(off)
+  const int cl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cl >= c.ncl) return;
+  const int si = c.csys[cl];
+  const BSys& s = sys[si];
+  const double4* old = par[si] ? posB : posA;
+  double4* nw = par[si] ? posA : posB;
+  const int sp = spar[si];
+  double4* vel = sta.vel[sp];
+  const double4* ref = sta.ref[sp];
+  const int k0 = c.coff[cl], K = c.coff[cl + 1] - k0, a0 = c.aoff[cl], na = c.aoff[cl + 1] - a0;
+  const int* la = c.la + k0;
+  const int* lb = c.lb + k0;
+  const double* len = c.len + k0;
+  double* W = c.work + 8 * (size_t)k0;
+  double* D = c.disp + 4 * (size_t)a0;
+  double* M = c.mat + c.moff[cl];
+  double* rhs = M + K * K;  // right-hand side, then the Newton step
+  for (int j = 0; j < na; ++j) {
+    const int t = c.islot[c.uatom[a0 + j]];
+    D[4 * j + 3] = 1.0 / s.mass[(int)old[t].w];
+  }
+  for (int k = 0; k < K; ++k) {
+    const int ta = c.islot[c.uatom[a0 + la[k]]], tb = c.islot[c.uatom[a0 + lb[k]]];
+    const double4 pa = old[ta], pb = old[tb], qa = nw[ta], qb = nw[tb];
+    double* w = W + 8 * k;
+    w[0] = mi_np(pa.x - pb.x, s.L[0]);
+    w[1] = mi_np(pa.y - pb.y, s.L[1]);
+    w[2] = mi_np(pa.z - pb.z, s.L[2]);
+    w[3] = mi_np(qa.x - qb.x, s.L[0]);
+    w[4] = mi_np(qa.y - qb.y, s.L[1]);
+    w[5] = mi_np(qa.z - qb.z, s.L[2]);
+    w[6] = 0.0;
+  }
+  cluster_disp(la, lb, W, D, K, na);
+  double best = INFINITY;
+  int stall = 0, it = 0;
+  for (; it < c.max_iter; ++it) {
+    // bond vectors of the displaced cluster: r_k = r0_k + D[a_k] - D[b_k]
+    double res = 0.0;
+    for (int k = 0; k < K; ++k) {
+      double* w = W + 8 * k;
+      const int a = la[k], b = lb[k];
+      const double rx = w[3] + (D[4 * a] - D[4 * b]), ry = w[4] + (D[4 * a + 1] - D[4 * b + 1]),
+                   rz = w[5] + (D[4 * a + 2] - D[4 * b + 2]);
+      const double L2 = len[k] * len[k];
+      rhs[k] = L2 - (rx * rx + ry * ry + rz * rz);
+      res = fmax(res, fabs(rhs[k]) / L2);
+      for (int l = 0; l < K; ++l) {
+        const double* o = W + 8 * l;
+        M[k * K + l] = 2.0 * coupling_kl(la, lb, D, k, l) * (rx * o[0] + ry * o[1] + rz * o[2]);
+      }
+    }
+    if (res <= c.tol) break;
+    if (res < 0.5 * best) {
+      best = res;
+      stall = 0;
+    } else if (++stall >= 10) {
+      if (atomicCAS(&c.cerr[0], 0, 2) == 0) {
+        c.cerr[1] = cl;
+        c.cres[0] = res;
+      }
+      return;
+    }
+    if (!gauss_solve(M, rhs, K)) {
+      if (atomicCAS(&c.cerr[0], 0, 2) == 0) {
+        c.cerr[1] = cl;
+        c.cres[0] = res;
+      }
+      return;
+    }
+    for (int k = 0; k < K; ++k) W[8 * k + 6] += rhs[k];
+    cluster_disp(la, lb, W, D, K, na);
+  }
+  if (it == c.max_iter) {
+    if (atomicCAS(&c.cerr[0], 0, 1) == 0) c.cerr[1] = cl;
+    return;
+  }
+  bool far = false;
+  for (int j = 0; j < na; ++j) {
+    const int t = c.islot[c.uatom[a0 + j]];
+    double4 x = nw[t], v = vel[t];
+    x.x = np_mod2(x.x + D[4 * j], s.L[0]);
+    x.y = np_mod2(x.y + D[4 * j + 1], s.L[1]);
+    x.z = np_mod2(x.z + D[4 * j + 2], s.L[2]);
+    v.x = v.x + D[4 * j] / s.dt;
+    v.y = v.y + D[4 * j + 1] / s.dt;
+    v.z = v.z + D[4 * j + 2] / s.dt;
+    nw[t] = x;
+    vel[t] = v;
+    const double4 r = ref[t];
+    const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+    const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+    const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+    far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+  }
+  if (far) {
+    atomicOr(&flags[si], 1);
+    atomicOr(&flags[S], 1);
+  }
+  double dev = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double4 pa = nw[c.islot[c.uatom[a0 + la[k]]]], pb = nw[c.islot[c.uatom[a0 + lb[k]]]];
+    const double rx = mi_np(pa.x - pb.x, s.L[0]), ry = mi_np(pa.y - pb.y, s.L[1]), rz = mi_np(pa.z - pb.z, s.L[2]);
+    dev = fmax(dev, fabs(sqrt(rx * rx + ry * ry + rz * rz) - len[k]) / len[k]);
+  }
+  atomicMax(c.maxviol, (unsigned long long)__double_as_longlong(dev));
+}
+
+// RATTLE of every cluster after the closing kick (integrators.py:227-234): one solve of
+// (C[k][l] r_k . r_l) mu = r_k . (v_a - v_b) removes the bond-stretching velocities,
+// v_a -= inv_a mu r, v_b += inv_b mu r per bond. With rec, the change of the recorded sums
+// (kinetic energy, momentum, angular momentum, as the fused launch forms them) per cluster
+// goes to cdelta, for k_b_cdelta to add to the system's partials before the record.
+__global__ void k_b_rattle(const BSys* __restrict__ sys, const double4* __restrict__ posA,
+                           const double4* __restrict__ posB, const int* __restrict__ par, BState sta,
+                           const int* __restrict__ spar, BCons c, int rec) {
+#pragma clang fp contract(off)
+  const int cl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cl >= c.ncl) return;
+  const int si = c.csys[cl];
+  const BSys& s = sys[si];
+  const double4* pos = par[si] ? posB : posA;
+  double4* vel = sta.vel[spar[si]];
+  const int k0 = c.coff[cl], K = c.coff[cl + 1] - k0, a0 = c.aoff[cl], na = c.aoff[cl + 1] - a0;
+  const int* la = c.la + k0;
+  const int* lb = c.lb + k0;
+  double* W = c.work + 8 * (size_t)k0;
+  double* D = c.disp + 4 * (size_t)a0;
+  double* M = c.mat + c.moff[cl];
+  double* rhs = M + K * K;
+  for (int j = 0; j < na; ++j) D[4 * j + 3] = 1.0 / s.mass[(int)pos[c.islot[c.uatom[a0 + j]]].w];
+  for (int k = 0; k < K; ++k) {
+    const int ta = c.islot[c.uatom[a0 + la[k]]], tb = c.islot[c.uatom[a0 + lb[k]]];
+    const double4 pa = pos[ta], pb = pos[tb], va = vel[ta], vb = vel[tb];
+    double* w = W + 8 * k;
+    w[0] = mi_np(pa.x - pb.x, s.L[0]);
+    w[1] = mi_np(pa.y - pb.y, s.L[1]);
+    w[2] = mi_np(pa.z - pb.z, s.L[2]);
+    rhs[k] = w[0] * (va.x - vb.x) + w[1] * (va.y - vb.y) + w[2] * (va.z - vb.z);
+  }
+  for (int k = 0; k < K; ++k)
+    for (int l = 0; l < K; ++l) {
+      const double *wk = W + 8 * k, *wl = W + 8 * l;
+      M[k * K + l] = coupling_kl(la, lb, D, k, l) * (wk[0] * wl[0] + wk[1] * wl[1] + wk[2] * wl[2]);
+    }
+  if (!gauss_solve(M, rhs, K)) {
+    if (atomicCAS(&c.cerr[0], 0, 3) == 0) c.cerr[1] = cl;
+    return;
+  }
+  for (int k = 0; k < K; ++k) W[8 * k + 6] = -rhs[k];  // v_a -= inv_a mu r: the displacement rule with G = -mu
+  cluster_disp(la, lb, W, D, K, na);
+  double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < na; ++j) {
+    const int t = c.islot[c.uatom[a0 + j]];
+    const double4 v = vel[t];
+    double4 u = v;
+    u.x = v.x + D[4 * j];
+    u.y = v.y + D[4 * j + 1];
+    u.z = v.z + D[4 * j + 2];
+    vel[t] = u;
+    if (rec) {
+      const double4 x = pos[t];
+      const double m = s.mass[(int)x.w];
+      const double px = m * v.x, py = m * v.y, pz = m * v.z, qx = m * u.x, qy = m * u.y, qz = m * u.z;
+      acc[1] += 0.5 * m * ((u.x * u.x + u.z * u.z) + u.y * u.y) - 0.5 * m * ((v.x * v.x + v.z * v.z) + v.y * v.y);
+      acc[2] += qx - px;
+      acc[3] += qy - py;
+      acc[4] += qz - pz;
+      acc[5] += (x.y * qz - x.z * qy) - (x.y * pz - x.z * py);
+      acc[6] += (x.z * qx - x.x * qz) - (x.z * px - x.x * pz);
+      acc[7] += (x.x * qy - x.y * qx) - (x.x * py - x.y * px);
+    }
+  }
+  if (rec)
+    for (int k = 0; k < kRecP; ++k) c.cdelta[(size_t)cl * kRecP + k] = acc[k];
+}
+
+// Per system (one block each): the clusters' record changes summed in cluster order and
+// added to the partials of the system's first brick, ahead of the record launch.
+__global__ void k_b_cdelta(const BSys* __restrict__ sys, BCons c, double* __restrict__ partials) {
+  __shared__ double red[kRecP * 4];
+  const int si = blockIdx.x;
+  double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = c.csoff[si] + threadIdx.x; q < c.csoff[si + 1]; q += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < kRecP; ++k) acc[k] += c.cdelta[(size_t)q * kRecP + k];
+  block_sum<kRecP>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRecP; ++k) partials[(size_t)sys[si].brick_off * kRecP + k] += acc[k];
+}
+
 // ------------------------------------------------------------------ host engine
 
 // The dynamic-LDS limit of every instantiation of the fused kernel.
@@ -1288,6 +1583,10 @@ struct BMD {
   long long* ndrift = nullptr;  // per system: drifts done (noise counter of the stochastic integrators)
   double* nh = nullptr;         // per system: Nose-Hoover xi, eta, Q, velocity scales s1, s2, k2
   BInteg ig = {kVV, 0.0, 0.0, 0.0, 0ull};
+  // rigid bonds (Rattle): clusters and their device buffers (freed with cbuf)
+  BCons cons = {};
+  int ncl = 0;
+  std::vector<void*> cbuf;
   int* bflags = nullptr;   // rebuild flags handed to the list launch
   unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
   int* bsum = nullptr;      // per-block sums of the rebuild scan
@@ -1323,6 +1622,7 @@ struct BMD {
                   bar, bsum, tphase, snap, snap_count, snap_rec, ndrift, nh};
     for (void* p : ps)
       if (p) (void)hipFree(p);
+    for (void* p : cbuf) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1367,6 +1667,7 @@ struct BMD {
     p.bflags = bflags;
     p.nreb = nreb;
     p.ndrift = ndrift;
+    p.islot = ncl ? cons.islot : nullptr;
     p.flip_n = flip_n;
     p.rec = rec;
     p.partials = partials;
@@ -1468,29 +1769,44 @@ struct BMD {
                        wn, flags, S, ig, ndrift, nh, drift);
   }
 
+  void launch_shake() {
+    hipLaunchKernelGGL(k_b_shake, dim3((ncl + 63) / 64), dim3(64), 0, st, d_sys, pos, pos_b, par, sta, spar, cons,
+                       flags, S);
+  }
+
+  void launch_rattle(int rec) {
+    hipLaunchKernelGGL(k_b_rattle, dim3((ncl + 63) / 64), dim3(64), 0, st, d_sys, pos, pos_b, par, sta, spar, cons,
+                       rec);
+    if (rec) hipLaunchKernelGGL(k_b_cdelta, dim3(S), dim3(256), 0, st, d_sys, cons, partials);
+  }
+
   void launch_nh(int end, int start, int rec) {
     hipLaunchKernelGGL(k_b_nh, dim3(S), dim3(kBT), 0, st, d_sys, partials, nh, ig, end, start, rec, log, log_cap,
                        log_count, nreb);
   }
 
-  // The thermostatted integrators (one step per coarse step, every system active): the
-  // fused launch stops after the forces and the closing kick (none for Brownian, whose
-  // velocities are dx/dt), the drift launch does the rest of the step, and Nose-Hoover
-  // adds its half steps in between (which also write its records).
+  // The thermostatted integrators and Rattle (one step per coarse step, every system
+  // active): the fused launch stops after the forces and the closing kick (none for
+  // Brownian, whose velocities are dx/dt), the drift launch does the rest of the step,
+  // Nose-Hoover adds its half steps in between (which also write its records), and
+  // Rattle its velocity solve after the closing kick and its SHAKE after the drift.
   void enqueue_thermo(int coarse, int record_every) {
-    const bool nose = ig.mode == kNoseHoover;
+    const bool nose = ig.mode == kNoseHoover, rat = ig.mode == kRattle;
     const double wc = ig.mode == kBrownian ? 0.0 : 1.0;
     if (nose) launch_nh(0, 1, 0);
     launch_drift(1.0, 1);
+    if (rat) launch_shake();
     launch_rebuild(S);
     for (int c = 0; c < coarse; ++c) {
       const bool last = c == coarse - 1;
       const int rec = record_every > 0 && ((c + 1) % record_every == 0 || last);
       launch_fused(nbricks, wc, 0.0, 0, nose ? 1 : rec);
+      if (rat) launch_rattle(rec);
       if (nose) launch_nh(1, !last, rec);
       else if (rec) launch_record();
       if (nose) launch_drift(1.0, !last);
       else if (!last) launch_drift(1.0, 1);
+      if (rat && !last) launch_shake();
       if (!last) launch_rebuild(S);
     }
   }
@@ -1896,7 +2212,8 @@ int accu_bmd_set_integrator(accu_bmd* h, int mode, double temperature, double fr
                             unsigned long long seed) {
   ARG_CHECK(h, "null handle");
   BMD& m = *reinterpret_cast<BMD*>(h);
-  ARG_CHECK(mode >= kVV && mode <= kNoseHoover, "unknown integrator mode %d", mode);
+  ARG_CHECK(mode >= kVV && mode <= kRattle, "unknown integrator mode %d", mode);
+  ARG_CHECK(mode != kRattle || m.ncl > 0, "Rattle needs constraints (accu_bmd_set_constraints)");
   ARG_CHECK(mode == kVV || temperature >= 0, "negative temperature");
   ARG_CHECK(mode != kBrownian || temperature > 0, "Brownian dynamics needs a positive temperature");
   ARG_CHECK(mode != kLangevin || friction >= 0, "negative friction");
@@ -1937,6 +2254,94 @@ int accu_bmd_get_thermostat(accu_bmd* h, double* out) {
   return ACCU_OK;
 }
 
+int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* aoff, const int* la, const int* lb,
+                             const double* len, const int* uatom, const int* csys, double tol, int max_iter) {
+  ARG_CHECK(h && coff && aoff && la && lb && len && uatom && csys, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(!m.ready, "set constraints before accu_bmd_init");
+  ARG_CHECK(m.ncl == 0, "constraints are already set");
+  ARG_CHECK(ncl >= 1, "no constraint clusters");
+  ARG_CHECK(tol > 0 && max_iter >= 1, "bad tolerance or iteration limit");
+  ARG_CHECK(coff[0] == 0 && aoff[0] == 0, "offsets must start at 0");
+  std::vector<long long> sys_lo(m.S + 1, 0);
+  for (int s = 0; s < m.S; ++s) sys_lo[s + 1] = sys_lo[s] + m.spec[s].natoms;
+  std::vector<int> csoff(m.S + 1, 0);
+  std::vector<long long> moff(ncl);
+  std::vector<char> seen(m.N, 0);
+  long long mtot = 0;
+  for (int c = 0; c < ncl; ++c) {
+    const int K = coff[c + 1] - coff[c], na = aoff[c + 1] - aoff[c], s = csys[c];
+    ARG_CHECK(K >= 1 && K <= kMaxClusterBonds, "cluster %d has %d bonds (1 to %d are supported)", c, K,
+              kMaxClusterBonds);
+    ARG_CHECK(na >= 2 && na <= K + 1, "cluster %d: %d atoms for %d bonds", c, na, K);
+    ARG_CHECK(s >= 0 && s < m.S && (c == 0 || s >= csys[c - 1]), "clusters must be ordered by system");
+    for (int j = aoff[c]; j < aoff[c + 1]; ++j) {
+      const int u = uatom[j];
+      ARG_CHECK(u >= sys_lo[s] && u < sys_lo[s + 1], "cluster %d: atom %d is not in system %d", c, u, s);
+      ARG_CHECK(!seen[u], "atom %d is in two clusters", u);
+      seen[u] = 1;
+    }
+    for (int k = coff[c]; k < coff[c + 1]; ++k)
+      ARG_CHECK(la[k] >= 0 && la[k] < na && lb[k] >= 0 && lb[k] < na && la[k] != lb[k] && len[k] > 0,
+                "cluster %d: bad bond %d", c, k);
+    moff[c] = mtot;
+    mtot += (long long)K * K + K;
+    for (int q = s + 1; q <= m.S; ++q) csoff[q] = c + 1;
+  }
+  const int nb = coff[ncl], na = aoff[ncl];
+  auto up = [&](const void* src, size_t bytes) -> void* {
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes + 64) != hipSuccess) return nullptr;
+    m.cbuf.push_back(d);
+    if (src && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    if (!src && hipMemset(d, 0, bytes + 64) != hipSuccess) return nullptr;
+    return d;
+  };
+  std::vector<int> ident(m.N);
+  for (int i = 0; i < m.N; ++i) ident[i] = i;
+  BCons& cs = m.cons;
+  cs.ncl = ncl;
+  cs.coff = (const int*)up(coff, (ncl + 1) * sizeof(int));
+  cs.aoff = (const int*)up(aoff, (ncl + 1) * sizeof(int));
+  cs.la = (const int*)up(la, nb * sizeof(int));
+  cs.lb = (const int*)up(lb, nb * sizeof(int));
+  cs.len = (const double*)up(len, nb * sizeof(double));
+  cs.uatom = (const int*)up(uatom, na * sizeof(int));
+  cs.csys = (const int*)up(csys, ncl * sizeof(int));
+  cs.csoff = (const int*)up(csoff.data(), (m.S + 1) * sizeof(int));
+  cs.moff = (const long long*)up(moff.data(), ncl * sizeof(long long));
+  cs.islot = (int*)up(ident.data(), m.N * sizeof(int));
+  cs.work = (double*)up(nullptr, (size_t)nb * 8 * sizeof(double));
+  cs.disp = (double*)up(nullptr, (size_t)na * 4 * sizeof(double));
+  cs.mat = (double*)up(nullptr, (size_t)mtot * sizeof(double));
+  cs.cdelta = (double*)up(nullptr, (size_t)ncl * kRecP * sizeof(double));
+  cs.cerr = (int*)up(nullptr, 2 * sizeof(int));
+  cs.cres = (double*)up(nullptr, sizeof(double));
+  cs.maxviol = (unsigned long long*)up(nullptr, sizeof(unsigned long long));
+  for (void* p : m.cbuf)
+    if (!p) {
+      set_error("device memory for %d constraint clusters", ncl);
+      return ACCU_ERUNTIME;
+    }
+  cs.tol = tol;
+  cs.max_iter = max_iter;
+  m.ncl = ncl;
+  return ACCU_OK;
+}
+
+int accu_bmd_get_constraints(accu_bmd* h, double* out, int* err) {
+  ARG_CHECK(h && out && err, "null argument");
+  BMD& m = *reinterpret_cast<BMD*>(h);
+  ARG_CHECK(m.ncl > 0, "no constraints are set");
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, m.cons.maxviol, sizeof bits, hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipMemcpyAsync(out + 1, m.cons.cres, sizeof(double), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipMemcpyAsync(err, m.cons.cerr, 2 * sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HIP_TRY(hipStreamSynchronize(m.st));
+  memcpy(out, &bits, sizeof(double));
+  return ACCU_OK;
+}
+
 int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const double* wc, const double* wn,
                  int record_every, int use_graph) {
   ARG_CHECK(h && active && wc && wn, "null argument");
@@ -1945,7 +2350,8 @@ int accu_bmd_run(accu_bmd* h, int coarse, int nticks, const int* active, const d
   ARG_CHECK(coarse >= 1 && nticks >= 1 && nticks <= 16, "bad step counts");
   ARG_CHECK(m.ig.mode == kVV || (nticks == 1 && wc[0] == 1.0 && wn[0] == 1.0),
             "the thermostatted integrators advance every system one step of dt per coarse step");
-  ARG_CHECK(!(m.ig.mode == kNoseHoover && m.snap), "snapshot frames are not taken with the Nose-Hoover integrator");
+  ARG_CHECK(!((m.ig.mode == kNoseHoover || m.ig.mode == kRattle) && m.snap),
+            "snapshot frames are not taken with the Nose-Hoover and Rattle integrators");
   for (int t = 0; t < nticks; ++t) ARG_CHECK(active[t] >= 1 && active[t] <= m.S, "bad active count at tick %d", t);
   ARG_CHECK(active[nticks - 1] == m.S, "the last tick must advance every system");
   std::vector<int> cnt(m.S);

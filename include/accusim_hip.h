@@ -173,8 +173,9 @@ int accu_bmd_timing(accu_bmd* h, double* total_ms_host, long long* launches_host
 /* integrator of every system of the arena: mode 0 velocity Verlet / Yoshida4 (the
  * accu_bmd_run weights), 1 Langevin BAOAB (integrators.py:50-71: temperature, friction
  * gamma), 2 Brownian (integrators.py:237-257: temperature, each species' diffusion from
- * accu_bmd_set_diffusion), 3 Nose-Hoover (integrators.py:74-116: temperature, period);
- * modes 1-3 run one step of dt per coarse step (accu_bmd_run with one tick, weights 1).
+ * accu_bmd_set_diffusion), 3 Nose-Hoover (integrators.py:74-116: temperature, period),
+ * 4 Rattle (integrators.py:119-234, bonds from accu_bmd_set_constraints);
+ * modes 1-4 run one step of dt per coarse step (accu_bmd_run with one tick, weights 1).
  * Langevin and Brownian noise: Philox4x32-10 normals keyed by seed, one counter per drift
  * of a system, one stream per atom user index. Nose-Hoover records carry the bath energy
  * (Q xi^2 / 2 + N_f kT eta, integrators.py:114-116) in log column 9. */
@@ -185,6 +186,21 @@ int accu_bmd_set_diffusion(accu_bmd* h, int sys, int ns, const double* D_host);
 /* Nose-Hoover state per system to out_host [nsys][6]: xi, eta, Q, the velocity scales of the
  * last two half steps, k2 */
 int accu_bmd_get_thermostat(accu_bmd* h, double* out_host);
+/* Rigid bonds for integrator mode 4, Rattle (integrators.py:119-234): SHAKE after each drift
+ * and RATTLE after each closing kick, both in float64, one thread per connected cluster of
+ * bonds (the reference's dense M-SHAKE system is block diagonal over these clusters).
+ * Before accu_bmd_init. Host arrays: cluster c has bonds [coff[c], coff[c+1]) with local
+ * atom indices la / lb in [0, na_c) and lengths len (m), atoms [aoff[c], aoff[c+1]) with
+ * global user indices uatom (system offsets included, each atom in at most one cluster),
+ * system csys[c] (clusters ordered by system); at most 512 bonds per cluster. tol and
+ * max_iter: the Newton stop |L^2 - r^2| / L^2 <= tol and iteration limit. */
+int accu_bmd_set_constraints(accu_bmd* h, int nclusters, const int* coff, const int* aoff, const int* la,
+                             const int* lb, const double* len, const int* uatom, const int* csys, double tol,
+                             int max_iter);
+/* out_host[0]: the worst relative bond-length error after any SHAKE so far (Rattle.max_violation),
+ * out_host[1]: the residual at a failure; err_host[0]: 0 ok, 1 SHAKE did not converge in
+ * max_iter, 2 SHAKE stalled (float64 cannot reach tol), 3 singular RATTLE system; err_host[1]: the cluster */
+int accu_bmd_get_constraints(accu_bmd* h, double* out_host, int* err_host);
 
 /* ---- analysis (accusim/analysis.py). pos / frames are device float64 arrays.
  * rdf pair counts (analysis.py:7-24): every pair i < j of pos [n][3] (minimum image when
