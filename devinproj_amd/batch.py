@@ -334,9 +334,11 @@ class BrickEngine:
         return ms.value, k.value
 
     def close(self):
-        if getattr(self, "_h", None) is not None and L._lib is not None:
-            L._lib.accu_bmd_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        lib = getattr(L, "_lib", None) if L is not None else None  # None at interpreter exit
+        if h is not None and lib is not None:
+            lib.accu_bmd_destroy(h)
+        self._h = None
 
     def __del__(self):
         self.close()

@@ -1244,8 +1244,8 @@ constexpr int kMaxClusterBonds = 512;  // bonds of one cluster (its solve is one
 // [aoff[c], aoff[c+1]) with global user indices uatom; clusters are ordered by system
 // (those of system s are [csoff[s], csoff[s+1])). islot[u] is the current slot of user
 // index u, written by every list rebuild. Work memory per cluster: 8 doubles per bond
-// (old bond vector, drifted bond vector, summed multiplier), 4 per atom (displacement,
-// inverse mass), K^2 + K at moff[c] (the Newton matrix and its right-hand side).
+// (old bond vector, drifted bond vector, summed multiplier), 5 per atom (displacement,
+// inverse mass, slot), K^2 + K at moff[c] (the Newton matrix and its right-hand side).
 struct BCons {
   int ncl;
   const int *coff, *aoff, *la, *lb, *uatom, *csys, *csoff;
@@ -1262,64 +1262,96 @@ struct BCons {
 
 __device__ __forceinline__ double mi_np(double d, double L) { return d - L * rint(d / L); }
 
+// A thread's view of one work array: element i at p[i * s]. Global work memory is
+// contiguous per cluster (s = 1); in the LDS the threads of a block interleave (s = 64,
+// element i of every lane in one row of 64 doubles), so lanes never share a bank.
+struct Scr {
+  double* p;
+  int s;
+  __device__ __forceinline__ double& operator()(int i) const { return p[i * s]; }
+};
+
+// LDS path: per-thread doubles for clusters of at most k bonds (8 per bond, 5 per atom,
+// k^2 + k for the Newton system); clusters up to kLdsBonds bonds fit 64 threads in 64 KiB.
+__host__ __device__ constexpr int cluster_doubles(int k) { return 8 * k + 5 * (k + 1) + k * k + k; }
+constexpr int kLdsBonds = 6;
+
 // A x = b for a K x K row-major A (overwritten) by Gaussian elimination with partial
 // pivoting; x replaces b. False for a singular A.
-__device__ bool gauss_solve(double* __restrict__ A, double* __restrict__ b, int K) {
+__device__ bool gauss_solve(Scr A, Scr b, int K) {
   for (int j = 0; j < K; ++j) {
     int p = j;
-    double best = fabs(A[j * K + j]);
+    double best = fabs(A(j * K + j));
     for (int i = j + 1; i < K; ++i)
-      if (fabs(A[i * K + j]) > best) {
-        best = fabs(A[i * K + j]);
+      if (fabs(A(i * K + j)) > best) {
+        best = fabs(A(i * K + j));
         p = i;
       }
     if (best == 0.0) return false;
     if (p != j) {
       for (int l = j; l < K; ++l) {
-        const double t = A[j * K + l];
-        A[j * K + l] = A[p * K + l];
-        A[p * K + l] = t;
+        const double t = A(j * K + l);
+        A(j * K + l) = A(p * K + l);
+        A(p * K + l) = t;
       }
-      const double t = b[j];
-      b[j] = b[p];
-      b[p] = t;
+      const double t = b(j);
+      b(j) = b(p);
+      b(p) = t;
     }
-    const double piv = A[j * K + j];
+    const double piv = A(j * K + j);
     for (int i = j + 1; i < K; ++i) {
-      const double f = A[i * K + j] / piv;
+      const double f = A(i * K + j) / piv;
       if (f == 0.0) continue;
-      for (int l = j + 1; l < K; ++l) A[i * K + l] -= f * A[j * K + l];
-      b[i] -= f * b[j];
+      for (int l = j + 1; l < K; ++l) A(i * K + l) -= f * A(j * K + l);
+      b(i) -= f * b(j);
     }
   }
   for (int j = K - 1; j >= 0; --j) {
-    double s = b[j];
-    for (int l = j + 1; l < K; ++l) s -= A[j * K + l] * b[l];
-    b[j] = s / A[j * K + j];
+    double s = b(j);
+    for (int l = j + 1; l < K; ++l) s -= A(j * K + l) * b(l);
+    b(j) = s / A(j * K + j);
   }
   return true;
 }
 
 // C[k][l] of a cluster: the change of bond vector k per unit multiplier on bond l, which
 // moves atom la[l] by +inv and lb[l] by -inv along that bond (integrators.py:202-205).
-__device__ __forceinline__ double coupling_kl(const int* la, const int* lb, const double* D, int k, int l) {
+__device__ __forceinline__ double coupling_kl(const int* la, const int* lb, Scr D, int k, int l) {
   const int ak = la[k], bk = lb[k], al = la[l], bl = lb[l];
-  const double ia = D[4 * ak + 3], ib = D[4 * bk + 3];
+  const double ia = D(5 * ak + 3), ib = D(5 * bk + 3);
   return ia * ((ak == al ? 1.0 : 0.0) - (ak == bl ? 1.0 : 0.0)) - ib * ((bk == al ? 1.0 : 0.0) - (bk == bl ? 1.0 : 0.0));
 }
 
 // D[j] = inv_j sum_l s(j, l) G_l ro_l: the displacement of every atom of the cluster for
 // summed multipliers G (s = +1 for the bond's first atom, -1 for its second).
-__device__ void cluster_disp(const int* la, const int* lb, const double* W, double* D, int K, int na) {
-  for (int j = 0; j < na; ++j) D[4 * j] = D[4 * j + 1] = D[4 * j + 2] = 0.0;
+__device__ void cluster_disp(const int* la, const int* lb, Scr W, Scr D, int K, int na) {
+  for (int j = 0; j < na; ++j) D(5 * j) = D(5 * j + 1) = D(5 * j + 2) = 0.0;
   for (int l = 0; l < K; ++l) {
-    const double* w = W + 8 * l;
     const int a = la[l], b = lb[l];
-    const double ga = w[6] * D[4 * a + 3], gb = w[6] * D[4 * b + 3];
+    const double g = W(8 * l + 6), ga = g * D(5 * a + 3), gb = g * D(5 * b + 3);
     for (int d = 0; d < 3; ++d) {
-      D[4 * a + d] += ga * w[d];
-      D[4 * b + d] -= gb * w[d];
+      const double w = W(8 * l + d);
+      D(5 * a + d) += ga * w;
+      D(5 * b + d) -= gb * w;
     }
+  }
+}
+
+// The work arrays of cluster cl: in the LDS (LDS: this lane's interleaved rows, laid out
+// for clusters of up to lds_k bonds) or in global memory. A compile-time choice, so each
+// access is an LDS or a global instruction rather than a generic one.
+template <bool LDS>
+__device__ __forceinline__ void cluster_scratch(const BCons& c, int cl, int k0, int a0, int lds_k, double* lds,
+                                                Scr& W, Scr& D, Scr& M) {
+  if constexpr (LDS) {
+    double* t = lds + (threadIdx.x & 63);
+    W = {t, 64};
+    D = {t + 64 * 8 * lds_k, 64};
+    M = {t + 64 * (8 * lds_k + 5 * (lds_k + 1)), 64};
+  } else {
+    W = {c.work + 8 * (size_t)k0, 1};
+    D = {c.disp + 5 * (size_t)a0, 1};
+    M = {c.mat + c.moff[cl], 1};
   }
 }
 
@@ -1330,12 +1362,12 @@ __device__ void cluster_disp(const int* la, const int* lb, const double* W, doub
 // velocities take the total displacement (velocities / dt), positions are wrapped again,
 // moved atoms are checked against the list's reference positions, and the worst relative
 // bond-length error is kept. A failure leaves the cluster as drifted and sets cerr.
-__global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ posA, double4* __restrict__ posB,
-                          const int* __restrict__ par, BState sta, const int* __restrict__ spar, BCons c,
-                          int* __restrict__ flags, int S) {
+// Returns the cluster's relative bond-length error (0 after a failure).
+template <bool LDS>
+__device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restrict__ posA, double4* __restrict__ posB,
+                                const int* __restrict__ par, const BState& sta, const int* __restrict__ spar,
+                                const BCons& c, int* __restrict__ flags, int S, int lds_k, double* cl_lds, int cl) {
 #pragma clang fp contract(off)
-  const int cl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cl >= c.ncl) return;
   const int si = c.csys[cl];
   const BSys& s = sys[si];
   const double4* old = par[si] ? posB : posA;
@@ -1347,25 +1379,24 @@ __global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ po
   const int* la = c.la + k0;
   const int* lb = c.lb + k0;
   const double* len = c.len + k0;
-  double* W = c.work + 8 * (size_t)k0;
-  double* D = c.disp + 4 * (size_t)a0;
-  double* M = c.mat + c.moff[cl];
-  double* rhs = M + K * K;  // right-hand side, then the Newton step
+  Scr W, D, M;
+  cluster_scratch<LDS>(c, cl, k0, a0, lds_k, cl_lds, W, D, M);
+  const Scr rhs = {&M(K * K), M.s};  // right-hand side, then the Newton step
   for (int j = 0; j < na; ++j) {
     const int t = c.islot[c.uatom[a0 + j]];
-    D[4 * j + 3] = 1.0 / s.mass[(int)old[t].w];
+    D(5 * j + 3) = 1.0 / s.mass[(int)old[t].w];
+    D(5 * j + 4) = (double)t;
   }
   for (int k = 0; k < K; ++k) {
-    const int ta = c.islot[c.uatom[a0 + la[k]]], tb = c.islot[c.uatom[a0 + lb[k]]];
+    const int ta = (int)D(5 * la[k] + 4), tb = (int)D(5 * lb[k] + 4);
     const double4 pa = old[ta], pb = old[tb], qa = nw[ta], qb = nw[tb];
-    double* w = W + 8 * k;
-    w[0] = mi_np(pa.x - pb.x, s.L[0]);
-    w[1] = mi_np(pa.y - pb.y, s.L[1]);
-    w[2] = mi_np(pa.z - pb.z, s.L[2]);
-    w[3] = mi_np(qa.x - qb.x, s.L[0]);
-    w[4] = mi_np(qa.y - qb.y, s.L[1]);
-    w[5] = mi_np(qa.z - qb.z, s.L[2]);
-    w[6] = 0.0;
+    W(8 * k) = mi_np(pa.x - pb.x, s.L[0]);
+    W(8 * k + 1) = mi_np(pa.y - pb.y, s.L[1]);
+    W(8 * k + 2) = mi_np(pa.z - pb.z, s.L[2]);
+    W(8 * k + 3) = mi_np(qa.x - qb.x, s.L[0]);
+    W(8 * k + 4) = mi_np(qa.y - qb.y, s.L[1]);
+    W(8 * k + 5) = mi_np(qa.z - qb.z, s.L[2]);
+    W(8 * k + 6) = 0.0;
   }
   cluster_disp(la, lb, W, D, K, na);
   double best = INFINITY;
@@ -1374,17 +1405,14 @@ __global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ po
     // bond vectors of the displaced cluster: r_k = r0_k + D[a_k] - D[b_k]
     double res = 0.0;
     for (int k = 0; k < K; ++k) {
-      double* w = W + 8 * k;
       const int a = la[k], b = lb[k];
-      const double rx = w[3] + (D[4 * a] - D[4 * b]), ry = w[4] + (D[4 * a + 1] - D[4 * b + 1]),
-                   rz = w[5] + (D[4 * a + 2] - D[4 * b + 2]);
+      const double rx = W(8 * k + 3) + (D(5 * a) - D(5 * b)), ry = W(8 * k + 4) + (D(5 * a + 1) - D(5 * b + 1)),
+                   rz = W(8 * k + 5) + (D(5 * a + 2) - D(5 * b + 2));
       const double L2 = len[k] * len[k];
-      rhs[k] = L2 - (rx * rx + ry * ry + rz * rz);
-      res = fmax(res, fabs(rhs[k]) / L2);
-      for (int l = 0; l < K; ++l) {
-        const double* o = W + 8 * l;
-        M[k * K + l] = 2.0 * coupling_kl(la, lb, D, k, l) * (rx * o[0] + ry * o[1] + rz * o[2]);
-      }
+      rhs(k) = L2 - (rx * rx + ry * ry + rz * rz);
+      res = fmax(res, fabs(rhs(k)) / L2);
+      for (int l = 0; l < K; ++l)
+        M(k * K + l) = 2.0 * coupling_kl(la, lb, D, k, l) * (rx * W(8 * l) + ry * W(8 * l + 1) + rz * W(8 * l + 2));
     }
     if (res <= c.tol) break;
     if (res < 0.5 * best) {
@@ -1395,34 +1423,37 @@ __global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ po
         c.cerr[1] = cl;
         c.cres[0] = res;
       }
-      return;
+      return 0.0;
     }
     if (!gauss_solve(M, rhs, K)) {
       if (atomicCAS(&c.cerr[0], 0, 2) == 0) {
         c.cerr[1] = cl;
         c.cres[0] = res;
       }
-      return;
+      return 0.0;
     }
-    for (int k = 0; k < K; ++k) W[8 * k + 6] += rhs[k];
+    for (int k = 0; k < K; ++k) W(8 * k + 6) += rhs(k);
     cluster_disp(la, lb, W, D, K, na);
   }
   if (it == c.max_iter) {
     if (atomicCAS(&c.cerr[0], 0, 1) == 0) c.cerr[1] = cl;
-    return;
+    return 0.0;
   }
   bool far = false;
   for (int j = 0; j < na; ++j) {
-    const int t = c.islot[c.uatom[a0 + j]];
+    const int t = (int)D(5 * j + 4);
     double4 x = nw[t], v = vel[t];
-    x.x = np_mod2(x.x + D[4 * j], s.L[0]);
-    x.y = np_mod2(x.y + D[4 * j + 1], s.L[1]);
-    x.z = np_mod2(x.z + D[4 * j + 2], s.L[2]);
-    v.x = v.x + D[4 * j] / s.dt;
-    v.y = v.y + D[4 * j + 1] / s.dt;
-    v.z = v.z + D[4 * j + 2] / s.dt;
+    x.x = np_mod2(x.x + D(5 * j), s.L[0]);
+    x.y = np_mod2(x.y + D(5 * j + 1), s.L[1]);
+    x.z = np_mod2(x.z + D(5 * j + 2), s.L[2]);
+    v.x = v.x + D(5 * j) / s.dt;
+    v.y = v.y + D(5 * j + 1) / s.dt;
+    v.z = v.z + D(5 * j + 2) / s.dt;
     nw[t] = x;
     vel[t] = v;
+    D(5 * j) = x.x;  // the displacement is applied: keep the final position for the check below
+    D(5 * j + 1) = x.y;
+    D(5 * j + 2) = x.z;
     const double4 r = ref[t];
     const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
     const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
@@ -1435,11 +1466,28 @@ __global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ po
   }
   double dev = 0.0;
   for (int k = 0; k < K; ++k) {
-    const double4 pa = nw[c.islot[c.uatom[a0 + la[k]]]], pb = nw[c.islot[c.uatom[a0 + lb[k]]]];
-    const double rx = mi_np(pa.x - pb.x, s.L[0]), ry = mi_np(pa.y - pb.y, s.L[1]), rz = mi_np(pa.z - pb.z, s.L[2]);
+    const int a = la[k], b = lb[k];
+    const double rx = mi_np(D(5 * a) - D(5 * b), s.L[0]), ry = mi_np(D(5 * a + 1) - D(5 * b + 1), s.L[1]),
+                 rz = mi_np(D(5 * a + 2) - D(5 * b + 2), s.L[2]);
     dev = fmax(dev, fabs(sqrt(rx * rx + ry * ry + rz * rz) - len[k]) / len[k]);
   }
-  atomicMax(c.maxviol, (unsigned long long)__double_as_longlong(dev));
+  return dev;
+}
+
+// Blocks of 64 threads (one wave); LDS: work arrays in the dynamic LDS (64 cluster_doubles(lds_k)).
+// The worst bond-length error of a wave takes one atomic.
+template <bool LDS>
+__global__ void __launch_bounds__(64) k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ posA,
+                                                double4* __restrict__ posB, const int* __restrict__ par, BState sta,
+                                                const int* __restrict__ spar, BCons c, int* __restrict__ flags, int S,
+                                                int lds_k) {
+  extern __shared__ double cl_lds[];
+  const int cl = blockIdx.x * blockDim.x + threadIdx.x;
+  double dev = 0.0;
+  if (cl < c.ncl) dev = shake_cluster<LDS>(sys, posA, posB, par, sta, spar, c, flags, S, lds_k, cl_lds, cl);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o, 64));
+  if (threadIdx.x == 0 && dev > 0.0) atomicMax(c.maxviol, (unsigned long long)__double_as_longlong(dev));
 }
 
 // RATTLE of every cluster after the closing kick (integrators.py:227-234): one solve of
@@ -1447,10 +1495,13 @@ __global__ void k_b_shake(const BSys* __restrict__ sys, double4* __restrict__ po
 // v_a -= inv_a mu r, v_b += inv_b mu r per bond. With rec, the change of the recorded sums
 // (kinetic energy, momentum, angular momentum, as the fused launch forms them) per cluster
 // goes to cdelta, for k_b_cdelta to add to the system's partials before the record.
-__global__ void k_b_rattle(const BSys* __restrict__ sys, const double4* __restrict__ posA,
-                           const double4* __restrict__ posB, const int* __restrict__ par, BState sta,
-                           const int* __restrict__ spar, BCons c, int rec) {
+template <bool LDS>
+__global__ void __launch_bounds__(64) k_b_rattle(const BSys* __restrict__ sys, const double4* __restrict__ posA,
+                                                 const double4* __restrict__ posB, const int* __restrict__ par,
+                                                 BState sta, const int* __restrict__ spar, BCons c, int rec,
+                                                 int lds_k) {
 #pragma clang fp contract(off)
+  extern __shared__ double cl_lds[];
   const int cl = blockIdx.x * blockDim.x + threadIdx.x;
   if (cl >= c.ncl) return;
   const int si = c.csys[cl];
@@ -1460,39 +1511,41 @@ __global__ void k_b_rattle(const BSys* __restrict__ sys, const double4* __restri
   const int k0 = c.coff[cl], K = c.coff[cl + 1] - k0, a0 = c.aoff[cl], na = c.aoff[cl + 1] - a0;
   const int* la = c.la + k0;
   const int* lb = c.lb + k0;
-  double* W = c.work + 8 * (size_t)k0;
-  double* D = c.disp + 4 * (size_t)a0;
-  double* M = c.mat + c.moff[cl];
-  double* rhs = M + K * K;
-  for (int j = 0; j < na; ++j) D[4 * j + 3] = 1.0 / s.mass[(int)pos[c.islot[c.uatom[a0 + j]]].w];
+  Scr W, D, M;
+  cluster_scratch<LDS>(c, cl, k0, a0, lds_k, cl_lds, W, D, M);
+  const Scr rhs = {&M(K * K), M.s};
+  for (int j = 0; j < na; ++j) {
+    const int t = c.islot[c.uatom[a0 + j]];
+    D(5 * j + 3) = 1.0 / s.mass[(int)pos[t].w];
+    D(5 * j + 4) = (double)t;
+  }
   for (int k = 0; k < K; ++k) {
-    const int ta = c.islot[c.uatom[a0 + la[k]]], tb = c.islot[c.uatom[a0 + lb[k]]];
+    const int ta = (int)D(5 * la[k] + 4), tb = (int)D(5 * lb[k] + 4);
     const double4 pa = pos[ta], pb = pos[tb], va = vel[ta], vb = vel[tb];
-    double* w = W + 8 * k;
-    w[0] = mi_np(pa.x - pb.x, s.L[0]);
-    w[1] = mi_np(pa.y - pb.y, s.L[1]);
-    w[2] = mi_np(pa.z - pb.z, s.L[2]);
-    rhs[k] = w[0] * (va.x - vb.x) + w[1] * (va.y - vb.y) + w[2] * (va.z - vb.z);
+    const double rx = mi_np(pa.x - pb.x, s.L[0]), ry = mi_np(pa.y - pb.y, s.L[1]), rz = mi_np(pa.z - pb.z, s.L[2]);
+    W(8 * k) = rx;
+    W(8 * k + 1) = ry;
+    W(8 * k + 2) = rz;
+    rhs(k) = rx * (va.x - vb.x) + ry * (va.y - vb.y) + rz * (va.z - vb.z);
   }
   for (int k = 0; k < K; ++k)
-    for (int l = 0; l < K; ++l) {
-      const double *wk = W + 8 * k, *wl = W + 8 * l;
-      M[k * K + l] = coupling_kl(la, lb, D, k, l) * (wk[0] * wl[0] + wk[1] * wl[1] + wk[2] * wl[2]);
-    }
+    for (int l = 0; l < K; ++l)
+      M(k * K + l) = coupling_kl(la, lb, D, k, l) *
+                     (W(8 * k) * W(8 * l) + W(8 * k + 1) * W(8 * l + 1) + W(8 * k + 2) * W(8 * l + 2));
   if (!gauss_solve(M, rhs, K)) {
     if (atomicCAS(&c.cerr[0], 0, 3) == 0) c.cerr[1] = cl;
     return;
   }
-  for (int k = 0; k < K; ++k) W[8 * k + 6] = -rhs[k];  // v_a -= inv_a mu r: the displacement rule with G = -mu
+  for (int k = 0; k < K; ++k) W(8 * k + 6) = -rhs(k);  // v_a -= inv_a mu r: the displacement rule with G = -mu
   cluster_disp(la, lb, W, D, K, na);
   double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int j = 0; j < na; ++j) {
-    const int t = c.islot[c.uatom[a0 + j]];
+    const int t = (int)D(5 * j + 4);
     const double4 v = vel[t];
     double4 u = v;
-    u.x = v.x + D[4 * j];
-    u.y = v.y + D[4 * j + 1];
-    u.z = v.z + D[4 * j + 2];
+    u.x = v.x + D(5 * j);
+    u.y = v.y + D(5 * j + 1);
+    u.z = v.z + D(5 * j + 2);
     vel[t] = u;
     if (rec) {
       const double4 x = pos[t];
@@ -1585,7 +1638,7 @@ struct BMD {
   BInteg ig = {kVV, 0.0, 0.0, 0.0, 0ull};
   // rigid bonds (Rattle): clusters and their device buffers (freed with cbuf)
   BCons cons = {};
-  int ncl = 0;
+  int ncl = 0, lds_k = 0;
   std::vector<void*> cbuf;
   int* bflags = nullptr;   // rebuild flags handed to the list launch
   unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
@@ -1769,14 +1822,18 @@ struct BMD {
                        wn, flags, S, ig, ndrift, nh, drift);
   }
 
+  // the SHAKE / RATTLE launches keep each cluster's work arrays in the LDS when the
+  // largest cluster has at most kLdsBonds bonds (lds_k = that size), else in global memory
+  size_t lds_cons() const { return lds_k ? (size_t)64 * cluster_doubles(lds_k) * sizeof(double) : 0; }
+
   void launch_shake() {
-    hipLaunchKernelGGL(k_b_shake, dim3((ncl + 63) / 64), dim3(64), 0, st, d_sys, pos, pos_b, par, sta, spar, cons,
-                       flags, S);
+    hipLaunchKernelGGL(lds_k ? k_b_shake<true> : k_b_shake<false>, dim3((ncl + 63) / 64), dim3(64), lds_cons(), st,
+                       d_sys, pos, pos_b, par, sta, spar, cons, flags, S, lds_k);
   }
 
   void launch_rattle(int rec) {
-    hipLaunchKernelGGL(k_b_rattle, dim3((ncl + 63) / 64), dim3(64), 0, st, d_sys, pos, pos_b, par, sta, spar, cons,
-                       rec);
+    hipLaunchKernelGGL(lds_k ? k_b_rattle<true> : k_b_rattle<false>, dim3((ncl + 63) / 64), dim3(64), lds_cons(), st,
+                       d_sys, pos, pos_b, par, sta, spar, cons, rec, lds_k);
     if (rec) hipLaunchKernelGGL(k_b_cdelta, dim3(S), dim3(256), 0, st, d_sys, cons, partials);
   }
 
@@ -2269,6 +2326,7 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
   std::vector<long long> moff(ncl);
   std::vector<char> seen(m.N, 0);
   long long mtot = 0;
+  int kmax = 0;
   for (int c = 0; c < ncl; ++c) {
     const int K = coff[c + 1] - coff[c], na = aoff[c + 1] - aoff[c], s = csys[c];
     ARG_CHECK(K >= 1 && K <= kMaxClusterBonds, "cluster %d has %d bonds (1 to %d are supported)", c, K,
@@ -2286,6 +2344,7 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
                 "cluster %d: bad bond %d", c, k);
     moff[c] = mtot;
     mtot += (long long)K * K + K;
+    kmax = std::max(kmax, K);
     for (int q = s + 1; q <= m.S; ++q) csoff[q] = c + 1;
   }
   const int nb = coff[ncl], na = aoff[ncl];
@@ -2312,7 +2371,7 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
   cs.moff = (const long long*)up(moff.data(), ncl * sizeof(long long));
   cs.islot = (int*)up(ident.data(), m.N * sizeof(int));
   cs.work = (double*)up(nullptr, (size_t)nb * 8 * sizeof(double));
-  cs.disp = (double*)up(nullptr, (size_t)na * 4 * sizeof(double));
+  cs.disp = (double*)up(nullptr, (size_t)na * 5 * sizeof(double));
   cs.mat = (double*)up(nullptr, (size_t)mtot * sizeof(double));
   cs.cdelta = (double*)up(nullptr, (size_t)ncl * kRecP * sizeof(double));
   cs.cerr = (int*)up(nullptr, 2 * sizeof(int));
@@ -2326,6 +2385,13 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
   cs.tol = tol;
   cs.max_iter = max_iter;
   m.ncl = ncl;
+  m.lds_k = kmax <= kLdsBonds ? kmax : 0;
+  if (m.lds_k) {
+    (void)hipFuncSetAttribute((const void*)k_b_shake<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)m.lds_cons());
+    (void)hipFuncSetAttribute((const void*)k_b_rattle<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)m.lds_cons());
+  }
   return ACCU_OK;
 }
 

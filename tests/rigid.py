@@ -2,7 +2,7 @@
 
 shape "dimer": 2 atoms, bond 1; "triangle": 3 atoms, all three sides 1 (a 3 x 3 cluster
 with a closed ring); "chain": 4 atoms in a planar zigzag with bonds 1 (a 3-bond path,
-flexible angles). Molecule centres sit on a cubic grid of the periodic box, positions are
+flexible angles); "chain8": the same zigzag of 8 atoms (7 bonds). Molecule centres sit on a cubic grid of the periodic box, positions are
 wrapped (bonds straddle the faces), velocities are a random rigid motion of each molecule
 (translation plus rotation about its centre), so they satisfy the constraints."""
 import numpy as np
@@ -13,6 +13,7 @@ SHAPES = {
     "dimer": (np.array([[-0.5, 0, 0], [0.5, 0, 0]]), [(0, 1)]),
     "triangle": (np.array([[0, 0, 0], [1, 0, 0], [0.5, np.sqrt(3) / 2, 0]]), [(0, 1), (1, 2), (0, 2)]),
     "chain": (np.array([[-1.2, -0.3, 0], [-0.4, 0.3, 0], [0.4, -0.3, 0], [1.2, 0.3, 0]]), [(0, 1), (1, 2), (2, 3)]),
+    "chain8": (np.array([[0.8 * i - 2.8, 0.3 * (-1) ** (i + 1), 0] for i in range(8)]), [(i, i + 1) for i in range(7)]),
 }
 
 
