@@ -247,6 +247,33 @@ def learned_throughput(reps=5):
             "note": "drop-in compute() wall times including host<->device copies"}
 
 
+def rattle_dimers(n_mol=131072, steps=200, block=20):
+    """Rigid bonds on the device: 131,072 rigid dimers (262,144 atoms, reduced LJ units,
+    eps 1, sigma 0.8, rho 0.2, fp64, dt 1e-3) with Rattle (SHAKE and RATTLE per bond
+    cluster, tol 1e-10) against velocity Verlet on the same fluid, hipGraph replays of
+    `block` steps, no records."""
+    import devinproj_amd as dp
+    from devinproj_amd import batch
+    from devinproj_amd import workloads as W
+    s, cons = W.rigid_dimers(n_mol, rho=0.2, seed=3)
+    out = {"atoms": s.n, "bonds": len(cons)}
+    for name, integ in (("vv", dp.VelocityVerlet()), ("rattle", dp.Rattle(cons, tol=1e-10))):
+        eng = batch.BrickEngine([s.copy()], [[dp.LennardJones({"A": (1.0, 0.8)}, cutoff=2.5, skin=0.3)]], integ,
+                                1e-3, log_cap=64)
+        eng.run(block, record_every=0, use_graph=True)
+        t0 = time.perf_counter()
+        for _ in range(steps // block):
+            eng.run(block, record_every=0, use_graph=True, sync=False)
+        eng.sync()
+        sec = time.perf_counter() - t0
+        eng.fetch(state=False)
+        out[f"{name}_ms_per_step"] = 1e3 * sec / (steps // block * block)
+        if name == "rattle":
+            out["max_violation"] = integ.max_violation
+        eng.close()
+    return out
+
+
 def copy_probe(mib=1024, reps=3):
     """Plain device-to-device copies of known size: the calibration kernel for the
     FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes `mib` MiB)."""
@@ -295,6 +322,7 @@ def main():
     ap.add_argument("--no-drift", action="store_true", help="skip the 10^4-step energy-drift runs")
     ap.add_argument("--no-richardson", action="store_true", help="skip the 48-system Richardson batch")
     ap.add_argument("--no-gravity", action="store_true", help="skip the Plummer gravity section")
+    ap.add_argument("--no-rattle", action="store_true", help="skip the rigid-dimer Rattle section")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -333,6 +361,8 @@ def main():
     log(f"richardson: {rich}")
     grav = None if args.no_gravity else gravity_plummer()
     log(f"gravity: {grav}")
+    rattle = None if args.no_rattle else rattle_dimers()
+    log(f"rattle: {rattle}")
     alg_bytes = n * 148 + 4 * (mixed["list_entries"] / 2)
     achieved = alg_bytes / (mixed["kernel_ms"] * 1e-3) / 1e9
     cpu = None
@@ -368,6 +398,7 @@ def main():
         "energy_drift_10k_steps": drift,
         "richardson_48_uneven_lj": rich,
         "plummer_262144_gravity": grav,
+        "rattle_131072_rigid_dimers": rattle,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": FUSED_TRAFFIC_BYTES,
                      "kernel": "k_b_fused<float,LJ>", "kernel_ms": mixed["kernel_ms"],

@@ -106,3 +106,25 @@ def uneven_batch(n_systems=48, seed=7, nmin=4000, nmax=262144):
     ns = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_systems))).astype(int)
     rhos = rng.uniform(0.7, 0.9, n_systems)
     return [(int(n), float(r), int(seed * 1000 + k)) for k, (n, r) in enumerate(zip(ns, rhos))]
+
+
+def rigid_dimers(n_mol, rho=0.2, bond=1.0, v_scale=0.5, seed=0):
+    """n_mol rigid dimers in reduced units (mass 1, bond `bond`) at number density rho: centres
+    on a cubic grid, random orientations, each molecule a random translation plus a rotation
+    about its centre (velocities that satisfy the bonds), zero net momentum. Returns
+    (System, constraints [(id_a, id_b, bond)]) for Rattle."""
+    rng = np.random.default_rng(seed)
+    L = (2 * n_mol / rho) ** (1 / 3)
+    m = int(np.ceil(n_mol ** (1 / 3)))
+    g = (np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n_mol] + 0.5) * (L / m)
+    u = rng.normal(size=(n_mol, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    w = rng.normal(0, v_scale, (n_mol, 3))
+    vc = rng.normal(0, v_scale, (n_mol, 3))
+    half = 0.5 * bond * u
+    s = System(3, box=L, periodic=True)
+    s.add_species("A", 1.0)
+    ia = s.add("A", (g - half) % L, vel=vc - np.cross(w, half))
+    ib = s.add("A", (g + half) % L, vel=vc + np.cross(w, half))
+    s.vel -= s.vel.mean(0)
+    return s, [(int(a), int(b), bond) for a, b in zip(ia, ib)]
