@@ -393,6 +393,8 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
                        -(1.0f + 0.5f * b.hd[2]) * h[2]};  // low corner of region cell 0
   const float rc2 = (float)(s.rc_list2 * (1.0 + 1e-5));
+  // z window half-width: a candidate with dz <= -rcw or dz >= rcw has dz^2 >= rc2 and fails the test anyway
+  const float rcw = sqrtf(rc2) * (1.0f + 1e-6f);
   const unsigned sentinel = (unsigned)total;
   const int* __restrict__ cell = p.st.cell[p.spar[b.sys]];
   const int a0 = p.start[b.cell0], a1 = p.start[b.cell0 + b.nhome];
@@ -411,30 +413,32 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
       for (int oy = 0; oy < 3; ++oy) {
         const int ry = ly + oy;
         const float ey = fmaxf(fmaxf(c0[1] + ry * h[1] - xi.y, xi.y - (c0[1] + (ry + 1) * h[1])), 0.f);
-        for (int oz = 0; oz < 3; ++oz) {
-          const int rz = lz + oz;
-          const float ez = fmaxf(fmaxf(c0[2] + rz * h[2] - xi.z, xi.z - (c0[2] + (rz + 1) * h[2])), 0.f);
-          if (ex * ex + ey * ey + ez * ez > rc2) continue;  // the cell's box is out of range
-          const int r = (rx * b.rd[1] + ry) * b.rd[2] + rz;
-          const int qe = rs[r + 1];
-          // four candidates per step: their LDS reads are in flight together (the slots
-          // past the cell's end re-read its last atom and are rejected by q < qe)
-          for (int q = rs[r]; q < qe; q += 4) {
-            float4 xj[4];
+        if (ex * ex + ey * ey > rc2) continue;  // the column's box is out of range
+        // the column's three cells (rz = lz .. lz + 2) are one run of slots sorted by z:
+        // scan only from near the first candidate with dz > -rcw (skipping 8 slots at a
+        // time past those below) to the first with dz >= rcw
+        const int r0 = (rx * b.rd[1] + ry) * b.rd[2] + lz;
+        int lo = rs[r0];
+        const int qe = rs[r0 + 3];
+        while (lo + 8 < qe && lds32[lo + 8].z - xi.z <= -rcw) lo += 8;
+        // four candidates per step: their LDS reads are in flight together (the slots
+        // past the run's end re-read its last atom and are rejected by q < qe)
+        for (int q = lo; q < qe; q += 4) {
+          float4 xj[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) xj[k] = lds32[min(q + k, qe - 1)];
+          for (int k = 0; k < 4; ++k) xj[k] = lds32[min(q + k, qe - 1)];
+          if (xj[0].z - xi.z >= rcw) break;  // this and every later candidate is above the window
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              const float dx = xj[k].x - xi.x, dy = xj[k].y - xi.y, dz = xj[k].z - xi.z;
-              if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != self) {
-                w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-                w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-                w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-                w3 = __builtin_amdgcn_alignbit((unsigned)(q + k), w3, 16);
-                if ((cnt & 7) == 7 && (cnt >> 3) < p.ngroups)
-                  p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
-                ++cnt;
-              }
+          for (int k = 0; k < 4; ++k) {
+            const float dx = xj[k].x - xi.x, dy = xj[k].y - xi.y, dz = xj[k].z - xi.z;
+            if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != self) {
+              w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+              w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+              w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+              w3 = __builtin_amdgcn_alignbit((unsigned)(q + k), w3, 16);
+              if ((cnt & 7) == 7 && (cnt >> 3) < p.ngroups)
+                p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+              ++cnt;
             }
           }
         }
@@ -458,6 +462,9 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
 
 // Parity of system si as seen inside the rebuild launch (the flip commits at its end).
 __device__ __forceinline__ int rb_par(const RB& p, int si) { return p.par[si] ^ (si < p.flip_n ? 1 : 0); }
+
+// Sort key of atom slot v of system si inside a cell: its current z.
+__device__ __forceinline__ double zkey(const RB& p, int v, int si) { return (rb_par(p, si) ? p.posB : p.posA)[v].z; }
 
 __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   const int tid = threadIdx.x, bd = blockDim.x;
@@ -559,9 +566,10 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   }
   if (!grid_sync(p.bar, p.ints + 2)) return;
   if (blockIdx.x == 0 && tid == 0) p.tphase[4] = wall_clock64();
-  // 4. rank each cell's atoms by original index and gather each atom's state to its
-  //    sorted slot in the other buffers; the list reference positions are the gathered
-  //    positions. Forces are not gathered: a rebuild always sits between a drift and
+  // 4. rank each cell's atoms by z (ties by original index) and gather each atom's state
+  //    to its sorted slot in the other buffers; the list reference positions are the
+  //    gathered positions. Cells stacked along z then form z-sorted runs of slots, which
+  //    the list build scans as one window per column. Forces are not gathered: a rebuild always sits between a drift and
   //    the fused step that recomputes every force of the system before any use. When both cells of a pair have at most 32 atoms each half-wave takes one; otherwise
   //    the whole wave takes the two in turn (shuffle rank up to 64 atoms, a rank against
   //    the cell's atoms in memory beyond that). Control flow stays wave-uniform.
@@ -590,8 +598,13 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       if (!__any(len > 32)) {
         const bool mine = on && hl < len;
         const int v = mine ? p.tmp[a + hl] : 0x7fffffff;
+        const double zv = mine ? zkey(p, v, p.cellsys[cm]) : INFINITY;
         int r = 0;
-        for (int x = 0; x < 32; ++x) r += (__shfl(v, (half << 5) + x, 64) < v) ? 1 : 0;
+        for (int x = 0; x < 32; ++x) {
+          const double zx = __shfl(zv, (half << 5) + x, 64);
+          const int vx = __shfl(v, (half << 5) + x, 64);
+          r += (zx < zv || (zx == zv && vx < v)) ? 1 : 0;
+        }
         if (mine) gather(cm, a + r, v, p.cellsys[cm]);
       } else {
         for (int k = 0; k < 2; ++k) {
@@ -603,11 +616,20 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
             const int e = e0 + lane;
             const bool mine = e < cl;
             const int v = mine ? p.tmp[ca + e] : 0x7fffffff;
+            const double zv = mine ? zkey(p, v, si) : INFINITY;
             int r = 0;
             if (cl <= 64) {
-              for (int x = 0; x < cl; ++x) r += (__shfl(v, x, 64) < v) ? 1 : 0;
+              for (int x = 0; x < cl; ++x) {
+                const double zx = __shfl(zv, x, 64);
+                const int vx = __shfl(v, x, 64);
+                r += (zx < zv || (zx == zv && vx < v)) ? 1 : 0;
+              }
             } else if (mine) {
-              for (int x = 0; x < cl; ++x) r += (p.tmp[ca + x] < v) ? 1 : 0;
+              for (int x = 0; x < cl; ++x) {
+                const int vx = p.tmp[ca + x];
+                const double zx = zkey(p, vx, si);
+                r += (zx < zv || (zx == zv && vx < v)) ? 1 : 0;
+              }
             }
             if (mine) gather(c, ca + r, v, si);
           }
