@@ -110,3 +110,20 @@ def test_device_rattle_argument_checks():
                             snapshots=4)
     with pytest.raises(ValueError, match="snapshot"):
         eng.run(2)
+
+
+def test_device_rattle_with_mixed_precision_forces():
+    """float32 pair forces (float64 sums), SHAKE / RATTLE still float64: bonds hold to the
+    same tolerance and the trajectory stays within the force rounding of the oracle's."""
+    s0, cons = fluid("dimer", 216, 0.2, seed=21)
+    s_o, _, _ = _oracle(s0, cons, 200, 1e-3, 1e-10)
+    s = s0.copy()
+    rat = dp.Rattle(cons, tol=1e-10)
+    eng = batch.BrickEngine([s], [[dp.LennardJones(LJ, cutoff=2.5, skin=0.3, precision="mixed")]], rat, 1e-3,
+                            precision="mixed")
+    eng.run(200, record_every=20)
+    eng.fetch()
+    assert rat.max_violation < 1e-9
+    assert _close(s.pos, s_o.pos, s.box) < 1e-5
+    e = np.array(eng.trajectory(0).kinetic) + np.array(eng.trajectory(0).potential)
+    assert np.max(np.abs(e - e[0])) / abs(e[0]) < 1e-3
