@@ -380,6 +380,7 @@ __device__ __forceinline__ const double4* cur_buf(const double4* A, const double
 // rc_list (a superset is harmless: forces apply their own cutoffs), so the test runs
 // in float32 on brick-relative coordinates with a 1e-5 relative margin on rc_list^2,
 // and region cells whose box is beyond rc_list from the atom are skipped.
+template <bool PK3>
 __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs, int* hself, int* wsum, int& mymax) {
   const BSys& s = p.sys[b.sys];
   const int total = region_setup(b, p.reg, p.start, rs, wsum, hself);
@@ -387,7 +388,8 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
     if (threadIdx.x == 0) atomicMax(&p.ints[1], total + 1);
     return;
   }
-  stage_region<float>(b, s, p.reg, p.start, rs, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
+  // PK3: 12-byte slots (the list test needs no species), so that more bricks share a CU's LDS
+  stage_region<float, PK3>(b, s, p.reg, p.start, rs, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
   __syncthreads();
   const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
@@ -401,7 +403,7 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
     const int local = cell[a] - b.cell0;
     const int self = hself[local] + a;
-    const float4 xi = lds32[self];
+    const float4 xi = ld_slot<PK3>(lds32, self);
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
     // 8 entries per 16-byte group: each hit shifts the 128-bit register down by 16 bits
     // and enters at the top, so the first entry of a group ends in the low half of w0
@@ -420,13 +422,13 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
         const int r0 = (rx * b.rd[1] + ry) * b.rd[2] + lz;
         int lo = rs[r0];
         const int qe = rs[r0 + 3];
-        while (lo + 8 < qe && lds32[lo + 8].z - xi.z <= -rcw) lo += 8;
+        while (lo + 8 < qe && ld_slot<PK3>(lds32, lo + 8).z - xi.z <= -rcw) lo += 8;
         // four candidates per step: their LDS reads are in flight together (the slots
         // past the run's end re-read its last atom and are rejected by q < qe)
         for (int q = lo; q < qe; q += 4) {
           float4 xj[4];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) xj[k] = lds32[min(q + k, qe - 1)];
+          for (int k = 0; k < 4; ++k) xj[k] = ld_slot<PK3>(lds32, min(q + k, qe - 1));
           if (xj[0].z - xi.z >= rcw) break;  // this and every later candidate is above the window
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
@@ -662,6 +664,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
 }
 
 // Verlet lists of the bricks of the systems the rebuild launch handed over.
+template <bool PK3>
 __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   if (p.bflags[p.S] == 0 || p.ints[2] != 0) return;
   const BBrick b = p.bricks[blockIdx.x];
@@ -671,7 +674,7 @@ __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   __shared__ int wsum[kBT / 64];
   extern __shared__ __align__(16) float4 lds32[];
   int mymax = 0;
-  build_brick(p, b, lds32, rs, hself, wsum, mymax);
+  build_brick<PK3>(p, b, lds32, rs, hself, wsum, mymax);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
   if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&p.ints[0], mymax);
@@ -1710,7 +1713,12 @@ struct BMD {
                         : (any_lj && !any_coul && one_sp) ? sizeof(Pk3) : sizeof(float4);
     return ((size_t)cap_fused * slot + 15) / 16 * 16;
   }
-  size_t lds_build() const { return (size_t)cap_build * sizeof(float4); }
+  // the list launch stages 12-byte slots when 16-byte ones would keep fewer than 4 bricks
+  // of 512 threads on a CU (160 KiB of LDS); 16-byte slots read in one instruction
+  bool pk3_build() const { return (size_t)cap_build * sizeof(float4) > 36 * 1024; }
+  size_t lds_build() const {
+    return pk3_build() ? ((size_t)cap_build * sizeof(Pk3) + 15) / 16 * 16 : (size_t)cap_build * sizeof(float4);
+  }
 
   RB rebuild_args(int bin_only, int flip_n, int rec = 0) const {
     RB p;
@@ -1775,7 +1783,8 @@ struct BMD {
   void launch_rebuild(int flip_n, int rec = 0) {
     const RB p = rebuild_args(0, flip_n, rec);
     hipLaunchKernelGGL(k_b_rebuild, dim3(grid_rb), dim3(kRB), 0, st, p);
-    hipLaunchKernelGGL(k_b_build, dim3(nbricks), dim3(threads), lds_build(), st, p);
+    hipLaunchKernelGGL(pk3_build() ? k_b_build<true> : k_b_build<false>, dim3(nbricks), dim3(threads), lds_build(),
+                       st, p);
   }
 
   template <typename P, bool ONE, bool EN>
@@ -2253,7 +2262,8 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   m.cap_fused = cap;
   ARG_CHECK(m.lds_build() + 6144 <= lds_max && m.lds_fused() + 6144 <= lds_max && cap < 65536,
             "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
-  (void)hipFuncSetAttribute((const void*)k_b_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
+  (void)hipFuncSetAttribute(m.pk3_build() ? (const void*)k_b_build<true> : (const void*)k_b_build<false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
   // neighbour capacity: measure with one group of 8, then size
   m.max_nbrs = 8;
   HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * sizeof(uint4)));
