@@ -1664,6 +1664,8 @@ struct BMD {
   // rigid bonds (Rattle): clusters and their device buffers (freed with cbuf)
   BCons cons = {};
   int ncl = 0, lds_k = 0;
+  std::vector<int> hc_coff, hc_aoff, hc_la, hc_lb, hc_uatom, hc_csys;  // host copies of the clusters
+  std::vector<double> hc_len;
   std::vector<void*> cbuf;
   int* bflags = nullptr;   // rebuild flags handed to the list launch
   unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
@@ -2189,6 +2191,8 @@ int accu_bmd_set_dsf(accu_bmd* h, int sys, double alpha, double rcut, double erc
 
 // Build tables, sort every system into brick order, build the lists, compute the
 // forces at t = 0 and record them. slack sizes the list and LDS capacities.
+static int order_clusters(BMD& m);
+
 int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   ARG_CHECK(h, "null handle");
   BMD& m = *reinterpret_cast<BMD*>(h);
@@ -2292,7 +2296,61 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   m.launch_record();
   LAUNCH_CHECK();
   HIP_TRY(hipStreamSynchronize(m.st));
+  if (m.ncl) {
+    rc = order_clusters(m);
+    if (rc) return rc;
+  }
   m.ready = true;
+  return ACCU_OK;
+}
+
+// After the set-up builds: the clusters of each system ordered by the slot of their first
+// atom, so that the 64 clusters of a wave in the SHAKE / RATTLE launches gather from
+// nearby slots (atoms stay near their cells between re-sorts, so the order stays close).
+static int order_clusters(BMD& m) {
+  std::vector<int> islot(m.N);
+  HIP_TRY(hipMemcpy(islot.data(), m.cons.islot, (size_t)m.N * sizeof(int), hipMemcpyDeviceToHost));
+  const int ncl = m.ncl;
+  std::vector<int> ord(ncl);
+  for (int c = 0; c < ncl; ++c) ord[c] = c;
+  auto key = [&](int c) { return std::make_pair(m.hc_csys[c], islot[m.hc_uatom[m.hc_aoff[c]]]); };
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key(a) < key(b); });
+  std::vector<int> coff(1, 0), aoff(1, 0), la, lb, uatom, csys;
+  std::vector<double> len;
+  std::vector<long long> moff;
+  long long mtot = 0;
+  for (int c : ord) {
+    for (int k = m.hc_coff[c]; k < m.hc_coff[c + 1]; ++k) {
+      la.push_back(m.hc_la[k]);
+      lb.push_back(m.hc_lb[k]);
+      len.push_back(m.hc_len[k]);
+    }
+    for (int j = m.hc_aoff[c]; j < m.hc_aoff[c + 1]; ++j) uatom.push_back(m.hc_uatom[j]);
+    coff.push_back((int)la.size());
+    aoff.push_back((int)uatom.size());
+    csys.push_back(m.hc_csys[c]);
+    const long long K = m.hc_coff[c + 1] - m.hc_coff[c];
+    moff.push_back(mtot);
+    mtot += K * K + K;
+  }
+  auto put = [&](const void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(const_cast<void*>(dst), src, bytes, hipMemcpyHostToDevice);
+  };
+  HIP_TRY(put(m.cons.coff, coff.data(), coff.size() * sizeof(int)));
+  HIP_TRY(put(m.cons.aoff, aoff.data(), aoff.size() * sizeof(int)));
+  HIP_TRY(put(m.cons.la, la.data(), la.size() * sizeof(int)));
+  HIP_TRY(put(m.cons.lb, lb.data(), lb.size() * sizeof(int)));
+  HIP_TRY(put(m.cons.len, len.data(), len.size() * sizeof(double)));
+  HIP_TRY(put(m.cons.uatom, uatom.data(), uatom.size() * sizeof(int)));
+  HIP_TRY(put(m.cons.csys, csys.data(), csys.size() * sizeof(int)));
+  HIP_TRY(put(m.cons.moff, moff.data(), moff.size() * sizeof(long long)));
+  m.hc_coff.swap(coff);
+  m.hc_aoff.swap(aoff);
+  m.hc_la.swap(la);
+  m.hc_lb.swap(lb);
+  m.hc_len.swap(len);
+  m.hc_uatom.swap(uatom);
+  m.hc_csys.swap(csys);
   return ACCU_OK;
 }
 
@@ -2417,6 +2475,13 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
   cs.tol = tol;
   cs.max_iter = max_iter;
   m.ncl = ncl;
+  m.hc_coff.assign(coff, coff + ncl + 1);
+  m.hc_aoff.assign(aoff, aoff + ncl + 1);
+  m.hc_la.assign(la, la + nb);
+  m.hc_lb.assign(lb, lb + nb);
+  m.hc_len.assign(len, len + nb);
+  m.hc_uatom.assign(uatom, uatom + na);
+  m.hc_csys.assign(csys, csys + ncl);
   m.lds_k = kmax <= kLdsBonds ? kmax : 0;
   if (m.lds_k) {
     (void)hipFuncSetAttribute((const void*)k_b_shake<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
