@@ -101,6 +101,15 @@ def test_device_rattle_failure_is_reported():
         eng.fetch()
 
 
+def test_device_rattle_iteration_limit_is_reported():
+    s0, cons = fluid("dimer", 64, 0.05, seed=2)
+    rat = dp.Rattle(cons, tol=1e-12, max_iter=1)
+    eng = batch.BrickEngine([s0.copy()], [[dp.LennardJones(LJ, cutoff=2.5, skin=0.3)]], rat, 1e-2)
+    eng.run(1)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        eng.fetch()
+
+
 def test_device_rattle_argument_checks():
     s0, cons = fluid("dimer", 64, 0.05, seed=2)
     with pytest.raises(ValueError, match="no constraint"):

@@ -89,3 +89,12 @@ def test_host_rattle_stall_raises():
     sim = A.Simulation(s0.copy(), [A.LennardJones({"A": (1.0, 0.8)}, cutoff=2.5)], integrator=rat, dt=1e-3)
     with pytest.raises(RuntimeError, match="stalled"):
         sim.run(steps=1, positions=False)
+
+
+def test_host_rattle_iteration_limit_raises():
+    # one Newton iteration cannot bring a drifted dimer fluid to 1e-12 (integrators.py:223-224)
+    s0, cons = fluid("dimer", 8, 0.05, seed=1)
+    rat = dp.Rattle(cons, tol=1e-12, max_iter=1)
+    sim = A.Simulation(s0.copy(), [A.LennardJones({"A": (1.0, 0.8)}, cutoff=2.5)], integrator=rat, dt=1e-2)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        sim.run(steps=1, positions=False)
