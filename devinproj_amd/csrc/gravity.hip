@@ -11,6 +11,51 @@ constexpr int kGT = 256;  // threads per block = j-tile size
 
 Box make_box(const double* box, int periodic);
 
+// One LDS tile of j-particles against target i. The self pair (j == i, excluded by the
+// reference's r^2 = inf on the diagonal) can only lie in a tile that holds a target of the
+// wave, so only those tiles (SELF, chosen wave-uniformly) pay the index compare and select
+// per pair; the others run without them.
+template <bool SELF>
+__device__ __forceinline__ void grav64_tile(const double4* tile, int cnt, int base, int i, double xi, double yi,
+                                            double zi, double eps2, const Box& box, double& fx, double& fy,
+                                            double& fz, double& phi) {
+  for (int k = 0; k < cnt; ++k) {
+    const double4 pj = tile[k];
+    double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+    if (box.periodic) {
+      dx = minimg_fast(dx, box.L[0], box.inv[0]);
+      dy = minimg_fast(dy, box.L[1], box.inv[1]);
+      dz = minimg_fast(dz, box.L[2], box.inv[2]);
+    }
+    const double r2 = r2_np(dx, dy, dz) + eps2;
+    double inv = rsqrt64(r2);
+    if constexpr (SELF) inv = (base + k == i) ? 0.0 : inv;
+    const double mi3 = pj.w * inv * inv * inv;
+    fx = fma(mi3, dx, fx);
+    fy = fma(mi3, dy, fy);
+    fz = fma(mi3, dz, fz);
+    phi = fma(pj.w, inv, phi);
+  }
+}
+
+template <bool SELF>
+__device__ __forceinline__ void grav32_tile(const float4* tile, int cnt, int base, int i, float xi, float yi,
+                                            float zi, float e2, float& gx, float& gy, float& gz, float& gp) {
+  for (int k = 0; k < cnt; ++k) {
+    const float4 pj = tile[k];
+    const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if constexpr (SELF) inv = (base + k == i) ? 0.f : inv;
+    const float mi = pj.w * inv;
+    const float mi3 = mi * inv * inv;
+    gx = fmaf(mi3, dx, gx);
+    gy = fmaf(mi3, dy, gy);
+    gz = fmaf(mi3, dz, gz);
+    gp += mi;
+  }
+}
+
 // FP64: every pair term and sum in float64.
 __global__ void __launch_bounds__(kGT) k_grav64(const double* __restrict__ pos, const double* __restrict__ m, int n,
                                                 const long long* __restrict__ tgt, int ntgt, double G, double eps2,
@@ -28,23 +73,10 @@ __global__ void __launch_bounds__(kGT) k_grav64(const double* __restrict__ pos, 
     if (j < n) tile[threadIdx.x] = make_double4(pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2], m[j]);
     __syncthreads();
     const int cnt = (n - base) < kGT ? (n - base) : kGT;
-    for (int k = 0; k < cnt; ++k) {
-      const double4 pj = tile[k];
-      double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-      if (box.periodic) {
-        dx = minimg_fast(dx, box.L[0], box.inv[0]);
-        dy = minimg_fast(dy, box.L[1], box.inv[1]);
-        dz = minimg_fast(dz, box.L[2], box.inv[2]);
-      }
-      const double r2 = r2_np(dx, dy, dz) + eps2;
-      double inv = rsqrt64(r2);
-      inv = (base + k == i) ? 0.0 : inv;
-      const double mi3 = pj.w * inv * inv * inv;
-      fx = fma(mi3, dx, fx);
-      fy = fma(mi3, dy, fy);
-      fz = fma(mi3, dz, fz);
-      phi = fma(pj.w, inv, phi);
-    }
+    if (__any(i >= base && i < base + cnt))
+      grav64_tile<true>(tile, cnt, base, i, xi, yi, zi, eps2, box, fx, fy, fz, phi);
+    else
+      grav64_tile<false>(tile, cnt, base, i, xi, yi, zi, eps2, box, fx, fy, fz, phi);
     __syncthreads();
   }
   double U[1] = {0.0};
@@ -81,19 +113,10 @@ __global__ void __launch_bounds__(kGT) k_grav32(const double* __restrict__ pos, 
     __syncthreads();
     const int cnt = (n - base) < kGT ? (n - base) : kGT;
     float gx = 0.f, gy = 0.f, gz = 0.f, gp = 0.f;
-    for (int k = 0; k < cnt; ++k) {
-      const float4 pj = tile[k];
-      const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
-      float inv = __builtin_amdgcn_rsqf(r2);
-      inv = (base + k == i) ? 0.f : inv;
-      const float mi = pj.w * inv;
-      const float mi3 = mi * inv * inv;
-      gx = fmaf(mi3, dx, gx);
-      gy = fmaf(mi3, dy, gy);
-      gz = fmaf(mi3, dz, gz);
-      gp += mi;
-    }
+    if (__any(i >= base && i < base + cnt))
+      grav32_tile<true>(tile, cnt, base, i, xi, yi, zi, e2, gx, gy, gz, gp);
+    else
+      grav32_tile<false>(tile, cnt, base, i, xi, yi, zi, e2, gx, gy, gz, gp);
     fx += (double)gx;
     fy += (double)gy;
     fz += (double)gz;
