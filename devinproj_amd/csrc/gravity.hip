@@ -8,6 +8,9 @@
 namespace accu {
 
 constexpr int kGT = 256;  // threads per block = j-tile size
+// direct_subset-sized calls (fewer target blocks than kSplitBelow, no energy) spread the
+// sources over about kSplitBlocks workgroups
+constexpr int kSplitBelow = 64, kSplitBlocks = 1024;
 
 Box make_box(const double* box, int periodic);
 
@@ -90,6 +93,53 @@ __global__ void __launch_bounds__(kGT) k_grav64(const double* __restrict__ pos, 
   if (Uout) grid_sum<1>(U, partials, counter, Uout, red);
 }
 
+// Few targets (direct_subset: 200 sampled stars against all of them), no energy: the
+// sources are split over `nsplit` ranges of whole tiles so that many workgroups share the
+// sum. Block b takes target block b % nb and source split b / nb, and writes its partial
+// force sums to part[(split * ntgt + r) * 3 + d]; k_grav_split_sum adds the splits in order.
+__global__ void __launch_bounds__(kGT) k_grav64_split(const double* __restrict__ pos, const double* __restrict__ m,
+                                                      int n, const long long* __restrict__ tgt, int ntgt, int nb,
+                                                      int tiles_per_split, double eps2, Box box,
+                                                      double* __restrict__ part) {
+  __shared__ double4 tile[kGT];
+  const int tb = blockIdx.x % nb, sp = blockIdx.x / nb;
+  const int r = tb * kGT + threadIdx.x;
+  const bool active = r < ntgt;
+  const int i = active ? (int)tgt[r] : 0;
+  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+  double fx = 0, fy = 0, fz = 0, phi = 0;
+  const int b0 = sp * tiles_per_split * kGT;
+  const int b1 = min(n, b0 + tiles_per_split * kGT);
+  for (int base = b0; base < b1; base += kGT) {
+    const int j = base + threadIdx.x;
+    if (j < n) tile[threadIdx.x] = make_double4(pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2], m[j]);
+    __syncthreads();
+    const int cnt = (n - base) < kGT ? (n - base) : kGT;
+    if (__any(i >= base && i < base + cnt))
+      grav64_tile<true>(tile, cnt, base, i, xi, yi, zi, eps2, box, fx, fy, fz, phi);
+    else
+      grav64_tile<false>(tile, cnt, base, i, xi, yi, zi, eps2, box, fx, fy, fz, phi);
+    __syncthreads();
+  }
+  if (active) {
+    double* o = part + ((size_t)sp * ntgt + r) * 3;
+    o[0] = fx;
+    o[1] = fy;
+    o[2] = fz;
+  }
+}
+
+__global__ void k_grav_split_sum(const double* __restrict__ m, const long long* __restrict__ tgt, int ntgt,
+                                 int nsplit, double G, const double* __restrict__ part, double* __restrict__ F) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ntgt) return;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int sp = 0; sp < nsplit; ++sp)
+    for (int d = 0; d < 3; ++d) f[d] += part[((size_t)sp * ntgt + r) * 3 + d];
+  const double gm = G * m[tgt[r]];
+  for (int d = 0; d < 3; ++d) F[3 * (size_t)r + d] = gm * f[d];
+}
+
 // Mixed: float32 pair terms summed per tile in float32, tile sums added in float64.
 // Positions are taken relative to `origin` (the centre of mass) before rounding.
 __global__ void __launch_bounds__(kGT) k_grav32(const double* __restrict__ pos, const double* __restrict__ m, int n,
@@ -153,7 +203,20 @@ int accu_gravity_direct(const double* pos, const double* mass, int n, const long
   const int nb = (ntgt + kGT - 1) / kGT;
   int rc = ws.ensure((size_t)nb * 2);
   if (rc) return rc;
-  if (precision == ACCU_FP64) {
+  const int ntiles = (n + kGT - 1) / kGT;
+  if (precision == ACCU_FP64 && targets && !U && nb < kSplitBelow && ntiles > 1) {
+    // few target blocks: split the sources so that about kSplitBlocks workgroups share them
+    const int want = std::max(1, std::min(ntiles, kSplitBlocks / nb));
+    const int tps = (ntiles + want - 1) / want;
+    const int nsplit = (ntiles + tps - 1) / tps;
+    rc = ws.ensure((size_t)nsplit * ntgt * 3);
+    if (rc) return rc;
+    Box b = make_box(box, periodic);
+    hipLaunchKernelGGL(k_grav64_split, dim3(nb * nsplit), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, nb, tps, eps2,
+                       b, ws.partials);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_grav_split_sum, dim3(nb), dim3(kGT), 0, st, mass, targets, ntgt, nsplit, G, ws.partials, F);
+  } else if (precision == ACCU_FP64) {
     Box b = make_box(box, periodic);
     hipLaunchKernelGGL(k_grav64, dim3(nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, b, F, ws.partials,
                        ws.counter, U);
