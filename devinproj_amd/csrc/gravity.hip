@@ -3,7 +3,10 @@
 //                  U   = -1/2 sum_{i != j} G m_i m_j / sqrt(r^2 + eps^2).
 // tree.py:81-87 (direct_subset) is the same sum restricted to target rows.
 #include "common.h"
+#include "gravity.h"
 #include "../../include/accusim_hip.h"
+
+#include <algorithm>
 
 namespace accu {
 
@@ -13,6 +16,21 @@ constexpr int kGT = 256;  // threads per block = j-tile size
 constexpr int kSplitBelow = 64, kSplitBlocks = 1024;
 
 Box make_box(const double* box, int periodic);
+
+// Particle of target row r: >= 0 for a valid row, -1 past the last row, -2 for a target
+// index outside [0, n) (its force row is written as NaN and nothing is read for it).
+__device__ __forceinline__ int target_of(const long long* tgt, int r, int ntgt, int n) {
+  if (r >= ntgt) return -1;
+  const long long t = tgt ? tgt[r] : r;
+  return (t >= 0 && t < n) ? (int)t : -2;
+}
+
+__device__ __forceinline__ void store_bad_row(double* F, int r) {
+  const double nan = __builtin_nan("");
+  F[3 * (size_t)r] = nan;
+  F[3 * (size_t)r + 1] = nan;
+  F[3 * (size_t)r + 2] = nan;
+}
 
 // One LDS tile of j-particles against target i. The self pair (j == i, excluded by the
 // reference's r^2 = inf on the diagonal) can only lie in a tile that holds a target of the
@@ -67,8 +85,10 @@ __global__ void __launch_bounds__(kGT) k_grav64(const double* __restrict__ pos, 
   __shared__ double4 tile[kGT];
   __shared__ double red[16];
   const int r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool active = r < ntgt;
-  const int i = active ? (tgt ? (int)tgt[r] : r) : 0;
+  const int t = target_of(tgt, r, ntgt, n);
+  const bool active = t >= 0;
+  const int i = active ? t : 0;
+  if (t == -2) store_bad_row(F, r);
   const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
   double fx = 0, fy = 0, fz = 0, phi = 0;
   for (int base = 0; base < n; base += kGT) {
@@ -104,8 +124,9 @@ __global__ void __launch_bounds__(kGT) k_grav64_split(const double* __restrict__
   __shared__ double4 tile[kGT];
   const int tb = blockIdx.x % nb, sp = blockIdx.x / nb;
   const int r = tb * kGT + threadIdx.x;
-  const bool active = r < ntgt;
-  const int i = active ? (int)tgt[r] : 0;
+  const int t = target_of(tgt, r, ntgt, n);
+  const bool active = t >= 0;
+  const int i = active ? t : 0;
   const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
   double fx = 0, fy = 0, fz = 0, phi = 0;
   const int b0 = sp * tiles_per_split * kGT;
@@ -129,14 +150,16 @@ __global__ void __launch_bounds__(kGT) k_grav64_split(const double* __restrict__
   }
 }
 
-__global__ void k_grav_split_sum(const double* __restrict__ m, const long long* __restrict__ tgt, int ntgt,
+__global__ void k_grav_split_sum(const double* __restrict__ m, int n, const long long* __restrict__ tgt, int ntgt,
                                  int nsplit, double G, const double* __restrict__ part, double* __restrict__ F) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ntgt) return;
+  const int t = target_of(tgt, r, ntgt, n);
+  if (t == -2) store_bad_row(F, r);
+  if (t < 0) return;
   double f[3] = {0.0, 0.0, 0.0};
   for (int sp = 0; sp < nsplit; ++sp)
     for (int d = 0; d < 3; ++d) f[d] += part[((size_t)sp * ntgt + r) * 3 + d];
-  const double gm = G * m[tgt[r]];
+  const double gm = G * m[t];
   for (int d = 0; d < 3; ++d) F[3 * (size_t)r + d] = gm * f[d];
 }
 
@@ -149,8 +172,10 @@ __global__ void __launch_bounds__(kGT) k_grav32(const double* __restrict__ pos, 
   __shared__ float4 tile[kGT];
   __shared__ double red[16];
   const int r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool active = r < ntgt;
-  const int i = active ? (tgt ? (int)tgt[r] : r) : 0;
+  const int t = target_of(tgt, r, ntgt, n);
+  const bool active = t >= 0;
+  const int i = active ? t : 0;
+  if (t == -2) store_bad_row(F, r);
   const float xi = (float)(pos[3 * (size_t)i] - ox), yi = (float)(pos[3 * (size_t)i + 1] - oy),
               zi = (float)(pos[3 * (size_t)i + 2] - oz);
   const float e2 = (float)eps2;
@@ -184,6 +209,58 @@ __global__ void __launch_bounds__(kGT) k_grav32(const double* __restrict__ pos, 
   if (Uout) grid_sum<1>(U, partials, counter, Uout, red);
 }
 
+// ---- launch planning shared by both entry points ---------------------------------
+
+GravPlan plan_gravity(int n, int ntgt, bool has_targets, bool has_U, int precision) {
+  GravPlan g;
+  g.nb = (ntgt + kGT - 1) / kGT;
+  const int ntiles = (n + kGT - 1) / kGT;
+  if (precision == ACCU_FP64 && has_targets && !has_U && g.nb < kSplitBelow && ntiles > 1) {
+    // few target blocks: split the sources so that about kSplitBlocks workgroups share them
+    const int want = std::max(1, std::min(ntiles, kSplitBlocks / g.nb));
+    g.tps = (ntiles + want - 1) / want;
+    g.nsplit = (ntiles + g.tps - 1) / g.tps;
+    g.need = (size_t)g.nsplit * ntgt * 3;
+  } else {
+    g.need = has_U ? (size_t)g.nb : 0;
+  }
+  return g;
+}
+
+// Enqueue the kernels of one call. partials holds g.need doubles; counter is the grid-sum
+// arrival counter (zero on entry, left zero by the sum).
+int launch_gravity(const GravPlan& g, const double* pos, const double* mass, int n, const long long* targets,
+                   int ntgt, double G, double eps2, const double* box, int periodic, int precision,
+                   const double* origin, double* F, double* U, double* partials, unsigned* counter, hipStream_t st) {
+  if (g.nsplit > 0) {
+    Box b = make_box(box, periodic);
+    hipLaunchKernelGGL(k_grav64_split, dim3(g.nb * g.nsplit), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, g.nb,
+                       g.tps, eps2, b, partials);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_grav_split_sum, dim3(g.nb), dim3(kGT), 0, st, mass, n, targets, ntgt, g.nsplit, G, partials,
+                       F);
+  } else if (precision == ACCU_FP64) {
+    Box b = make_box(box, periodic);
+    hipLaunchKernelGGL(k_grav64, dim3(g.nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, b, F, partials,
+                       counter, U);
+  } else {
+    double o[3] = {0, 0, 0};
+    if (origin)
+      for (int d = 0; d < 3; ++d) o[d] = origin[d];
+    hipLaunchKernelGGL(k_grav32, dim3(g.nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, o[0], o[1], o[2],
+                       F, partials, counter, U);
+  }
+  LAUNCH_CHECK();
+  return ACCU_OK;
+}
+
+int check_gravity_args(const double* pos, const double* mass, double* F, int precision, int periodic) {
+  ARG_CHECK(pos && mass && F, "null argument");
+  ARG_CHECK(precision == ACCU_FP64 || precision == ACCU_FP32_PAIR, "unknown precision %d", precision);
+  ARG_CHECK(!(precision == ACCU_FP32_PAIR && periodic), "float32 gravity supports open boundaries only");
+  return ACCU_OK;
+}
+
 }  // namespace accu
 
 using namespace accu;
@@ -193,42 +270,16 @@ extern "C" {
 int accu_gravity_direct(const double* pos, const double* mass, int n, const long long* targets, int ntgt, double G,
                         double eps2, const double* box, int periodic, int precision, const double* origin,
                         double* F, double* U, void* stream) {
-  ARG_CHECK(pos && mass && F, "null argument");
-  ARG_CHECK(precision == ACCU_FP64 || precision == ACCU_FP32_PAIR, "unknown precision %d", precision);
-  ARG_CHECK(!(precision == ACCU_FP32_PAIR && periodic), "float32 gravity supports open boundaries only");
+  int rc = check_gravity_args(pos, mass, F, precision, periodic);
+  if (rc) return rc;
   if (!targets) ntgt = n;
   if (n <= 0 || ntgt <= 0) return ACCU_OK;
-  hipStream_t st = (hipStream_t)stream;
+  const GravPlan g = plan_gravity(n, ntgt, targets != nullptr, U != nullptr, precision);
   Workspace& ws = default_workspace();
-  const int nb = (ntgt + kGT - 1) / kGT;
-  int rc = ws.ensure((size_t)nb * 2);
+  rc = ws.ensure(g.need);
   if (rc) return rc;
-  const int ntiles = (n + kGT - 1) / kGT;
-  if (precision == ACCU_FP64 && targets && !U && nb < kSplitBelow && ntiles > 1) {
-    // few target blocks: split the sources so that about kSplitBlocks workgroups share them
-    const int want = std::max(1, std::min(ntiles, kSplitBlocks / nb));
-    const int tps = (ntiles + want - 1) / want;
-    const int nsplit = (ntiles + tps - 1) / tps;
-    rc = ws.ensure((size_t)nsplit * ntgt * 3);
-    if (rc) return rc;
-    Box b = make_box(box, periodic);
-    hipLaunchKernelGGL(k_grav64_split, dim3(nb * nsplit), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, nb, tps, eps2,
-                       b, ws.partials);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_grav_split_sum, dim3(nb), dim3(kGT), 0, st, mass, targets, ntgt, nsplit, G, ws.partials, F);
-  } else if (precision == ACCU_FP64) {
-    Box b = make_box(box, periodic);
-    hipLaunchKernelGGL(k_grav64, dim3(nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, b, F, ws.partials,
-                       ws.counter, U);
-  } else {
-    double o[3] = {0, 0, 0};
-    if (origin)
-      for (int d = 0; d < 3; ++d) o[d] = origin[d];
-    hipLaunchKernelGGL(k_grav32, dim3(nb), dim3(kGT), 0, st, pos, mass, n, targets, ntgt, G, eps2, o[0], o[1], o[2],
-                       F, ws.partials, ws.counter, U);
-  }
-  LAUNCH_CHECK();
-  return ACCU_OK;
+  return launch_gravity(g, pos, mass, n, targets, ntgt, G, eps2, box, periodic, precision, origin, F, U, ws.partials,
+                        ws.counter, (hipStream_t)stream);
 }
 
 }  // extern "C"

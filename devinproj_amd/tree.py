@@ -48,7 +48,11 @@ class BarnesHut(Force):
 def direct_subset(force, s, sub):
     """Exact gravitational force on particles `sub` from all particles (tree.py:81-87), on the GPU."""
     th = L.torch()
-    sub = np.ascontiguousarray(sub, np.int64)
+    sub = np.ascontiguousarray(sub, np.int64).reshape(-1)
+    if len(sub) and (sub.min() < -s.n or sub.max() >= s.n):
+        bad = sub[(sub < -s.n) | (sub >= s.n)][0]
+        raise IndexError(f"index {bad} is out of bounds for axis 0 with size {s.n}")
+    sub = np.where(sub < 0, sub + s.n, sub)  # numpy's negative indices, as s.pos[sub] takes them
     buf = L.Buffers()
     pos_d = buf.upload("pos", np.asarray(s.pos, float), th.float64)
     m_d = buf.upload("m", np.asarray(s.masses, float), th.float64)

@@ -78,7 +78,8 @@ int accu_ewald_recip(const double* pos, const double* q, int n, const double* bo
 
 /* ---- Gravity: forces.py:38-48; with targets (device int64 [ntgt]) the rows of
  * tree.py:81-87 direct_subset (F is then [ntgt][3]). origin (host [3], may be NULL)
- * is subtracted before float32 rounding in ACCU_FP32_PAIR mode. */
+ * is subtracted before float32 rounding in ACCU_FP32_PAIR mode. Targets must lie in
+ * [0, n): the kernels read nothing for one outside and write its row of F as NaN. */
 int accu_gravity_direct(const double* pos, const double* mass, int n, const long long* targets, int ntgt, double G,
                         double eps2, const double* box_host, int periodic, int precision, const double* origin_host,
                         double* F, double* U, void* stream);

@@ -239,6 +239,32 @@ def test_gravity_matches_oracle_and_subset():
     assert A.rms_rel_force_error(Fm, Fo) < 1e-5 and abs(Um / Uo - 1) < 1e-6
 
 
+def test_direct_subset_and_gravity_targets_out_of_range():
+    # tree.py:81-87 indexes s.pos[sub]: out-of-range indices raise IndexError, negative
+    # ones count from the end, as numpy takes them
+    s, a, M = B.plummer(n=700, seed=6)
+    g = dp.Gravity(softening=0.01 * a)
+    with pytest.raises(IndexError):
+        dp.direct_subset(g, s, np.array([3, 700]))
+    with pytest.raises(IndexError):
+        dp.direct_subset(g, s, np.array([-701]))
+    Fneg = dp.direct_subset(g, s, np.array([-1, 5]))
+    assert np.array_equal(Fneg, dp.direct_subset(g, s, np.array([699, 5])))
+    # through the C ABI a target outside [0, n) reads nothing and comes back as a NaN row
+    th = dp._lib.torch()
+    L = dp._lib
+    pos = th.from_numpy(s.pos.copy()).cuda()
+    m = th.from_numpy(s.masses.copy()).cuda()
+    tgt = th.tensor([4, 700, -3, 9], dtype=th.int64, device="cuda")
+    F = th.zeros((4, 3), dtype=th.float64, device="cuda")
+    L.call("accu_gravity_direct", L.ptr(pos), L.ptr(m), s.n, L.ptr(tgt), 4, float(g.G), float(g.eps2), None, 0,
+           L.ACCU_FP64, None, L.ptr(F), None, L.stream())
+    Fh = F.cpu().numpy()
+    assert np.isnan(Fh[1]).all() and np.isnan(Fh[2]).all()
+    ref = A.direct_subset(g, s, np.array([4, 9]))
+    assert np.max(np.abs(Fh[[0, 3]] - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
 def test_barnes_hut_theta_zero_is_exact_and_matches_reference(golden):
     s, a, M = B.plummer(n=300, seed=1)
     F0, U0 = A.Gravity(softening=0.01 * a).compute(s)
