@@ -1,0 +1,138 @@
+"""The 1,048,576-atom argon workload at full size through the brick engine (csrc/brick.hip),
+in the tiles the benchmark runs (mixed 3x3x3 bricks, fp64 2x2x3, 512 threads): region
+capacities of thousands of slots, about 2,000 bricks and 16-bit region indices at their
+real sizes.
+
+Checks (the reference's LJ is forces.py:119-142, its cell list forces.py:279-321):
+  - the positions are the committed configuration (fingerprint from
+    tests/golden/make_c2_pairs.py);
+  - list entries lie between twice the oracle's pair counts within rc_list and within
+    rc_list (1 + 1e-5), both counted by the oracle's cell_list_pairs on this configuration;
+  - the float64 forces of 200 sampled atoms equal the oracle's LennardJones evaluated on
+    a periodic sub-system that holds every atom within the list radius of them (1e-10);
+  - mixed against float64: RMS relative force error <= 1e-5 over all atoms, potential
+    energy within 1e-6;
+  - net force and momentum over 50 velocity-Verlet steps: float64 keeps the total
+    momentum within 1e-9 of sum m|v| (the report's PASS grade, accuracy.py:87-91), mixed
+    within 1e-6 (its WARN bound)."""
+import numpy as np
+import pytest
+
+from oracle import accusim_np as A
+from oracle import builders as B
+
+pytestmark = pytest.mark.gpu
+
+dp = pytest.importorskip("devinproj_amd")
+from devinproj_amd import autotune, batch  # noqa: E402
+
+DT = 10e-15
+
+
+@pytest.fixture(scope="module")
+def workload(golden):
+    g = golden("argon1m_pairs.npz")
+    s = B.argon(64, seed=2, jitter=0.01)
+    assert s.n == int(g["n"]) == 1 << 20
+    assert np.array_equal(s.pos[:4], g["pos_head"]) and np.array_equal(s.vel[:4], g["vel_head"])
+    assert np.allclose(s.pos.sum(axis=0), g["pos_sum"], rtol=1e-13, atol=0)
+    return s, g
+
+
+@pytest.fixture(scope="module")
+def golden(request):
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return lambda name: np.load(os.path.join(here, name), allow_pickle=False)
+
+
+def _engine(s, precision):
+    lj = dp.LennardJones(B.argon_params(), cutoff=2.5, skin=0.3, precision=precision)
+    tile = autotune.cached_tile(s, [lj], precision, autotune.DEFAULT_CACHE)
+    assert tile is not None, "no tuned tile for the 1M-atom workload"
+    print(f"argon 1M {precision}: bricks {tile['bricks']} threads {tile['threads']}", flush=True)
+    return batch.BrickEngine([s], [[lj]], dp.VelocityVerlet(), DT, precision=precision, bricks=[tile["bricks"]],
+                             threads=tile["threads"], log_cap=128)
+
+
+def _neighbourhood(s, targets, radius):
+    """Indices of every atom within `radius` (minimum image) of any target, targets first."""
+    keep = np.zeros(s.n, bool)
+    L = s.box
+    for k in range(0, len(targets), 16):
+        t = targets[k:k + 16]
+        d = s.pos[None, :, :] - s.pos[t][:, None, :]
+        d -= L * np.round(d / L)
+        keep |= np.any(np.einsum("tjk,tjk->tj", d, d) < radius ** 2, axis=0)
+    keep[targets] = False
+    return np.concatenate([targets, np.flatnonzero(keep)])
+
+
+@pytest.fixture(scope="module")
+def state_at_setup(workload):
+    s, g = workload
+    out = {}
+    for precision in ("fp64", "mixed"):
+        eng = _engine(s.copy(), precision)
+        eng.fetch(state=True)
+        out[precision] = {"F": eng.forces_out[0].copy(), "U": float(eng.logs[0][0, 0]),
+                          "entries": eng.info["list_entries"], "region": eng.info["region_capacity"]}
+        eng.close()
+    return out
+
+
+def test_list_entries_bracket_oracle_pair_counts(workload, state_at_setup):
+    s, g = workload
+    lo, hi = 2 * int(g["n_lo"]), 2 * int(g["n_hi"])
+    for precision, st in state_at_setup.items():
+        print(f"{precision}: list entries {st['entries']} in [{lo}, {hi}], region capacity {st['region']}")
+        assert lo <= st["entries"] <= hi
+
+
+def test_fp64_forces_of_sampled_atoms_equal_oracle(workload, state_at_setup):
+    s, g = workload
+    targets = np.sort(np.random.default_rng(5).choice(s.n, 200, replace=False))
+    rc_list = float(g["rc_list"])
+    idx = _neighbourhood(s, targets, rc_list)
+    sub = A.System(3, box=s.box, periodic=True)
+    sub.add_species("Ar", s.species[0].mass)
+    sub.add("Ar", s.pos[idx])
+    Fo, _ = A.LennardJones(B.argon_params(), cutoff=2.5, skin=0.3).compute(sub)
+    Fg = state_at_setup["fp64"]["F"][targets]
+    scale = np.max(np.abs(Fo[:len(targets)]))
+    err = np.max(np.abs(Fg - Fo[:len(targets)])) / scale
+    print(f"sub-system of {sub.n} atoms; fp64 max force error {err:.2e}")
+    assert err < 1e-10
+    Fm = state_at_setup["mixed"]["F"][targets]
+    assert A.rms_rel_force_error(Fm, Fo[:len(targets)]) < 1e-5
+
+
+def test_mixed_against_fp64_forces_and_energy(state_at_setup):
+    F64, Fm = state_at_setup["fp64"]["F"], state_at_setup["mixed"]["F"]
+    U64, Um = state_at_setup["fp64"]["U"], state_at_setup["mixed"]["U"]
+    e = A.rms_rel_force_error(Fm, F64)
+    print(f"mixed vs fp64: RMS relative force error {e:.2e}, energy {Um / U64 - 1:.2e}")
+    assert e <= 1e-5
+    assert abs(Um / U64 - 1) < 1e-6
+
+
+@pytest.mark.parametrize("precision,p_tol,f_tol", [("fp64", 1e-9, 1e-12), ("mixed", 1e-6, 1e-6)])
+def test_net_force_and_momentum_over_50_steps(workload, state_at_setup, precision, p_tol, f_tol):
+    s, g = workload
+    F = state_at_setup[precision]["F"]
+    net = np.linalg.norm(F.sum(axis=0)) / np.sum(np.linalg.norm(F, axis=1))
+    sc = s.copy()
+    eng = _engine(sc, precision)
+    eng.run(50, record_every=1)
+    eng.fetch(state=True)
+    lg = eng.logs[0]
+    assert len(lg) == 51
+    m = s.masses
+    scale = float(np.sum(m * np.linalg.norm(s.vel, axis=1)))
+    dP = float(np.max(np.linalg.norm(lg[:, 2:5] - lg[0, 2:5], axis=1))) / scale
+    print(f"{precision}: net force {net:.2e}, momentum drift {dP:.2e}, rebuilds {int(eng.rebuilds[0])}, "
+          f"energy drift {eng.energy_drift():.2e}")
+    assert net < f_tol
+    assert dP < p_tol
+    assert np.all(np.isfinite(sc.pos)) and np.all((sc.pos >= 0) & (sc.pos <= s.box))
+    eng.close()
