@@ -32,7 +32,7 @@ def main(argv=None):
     record_every = max(1, steps // 200)
     s = sim.system
     print(f"{s.n} particles, {len(s.species)} species, dt = {sim.dt:.3g} s, {steps} steps, engine {a.engine}")
-    from .accuracy import AccuracyReport, Check, PASS, drift_rows
+    from .accuracy import AccuracyReport
     if a.engine == "dropin":
         tr = sim.run(steps=steps, record_every=record_every, positions=bool(a.save))
         report = AccuracyReport.build(sim, convergence_duration=duration if a.convergence else None)
@@ -43,8 +43,7 @@ def main(argv=None):
         eng.run(steps, record_every=record_every)
         eng.fetch()
         tr = eng.trajectory(0)
-        report = AccuracyReport([Check("units", PASS, None, "every input was dimension-checked on entry")] +
-                                drift_rows(tr, s, sim.forces, sim.integrator))
+        report = AccuracyReport.from_run(tr, s, sim.forces, sim.integrator, sim.dt)
         eng.close()
     print(report)
     if a.save:

@@ -177,6 +177,9 @@ def build_parts(model: dict, precision: str = "fp64"):
             raise ValueError(f"'{key}' are not on the GPU force path (run this model with accusim itself)")
     dim = model.get("dim", 3)
     box = si(model["box"], "length", "box") if "box" in model else None
+    if box is not None and not model.get("periodic", False):
+        raise ValueError("reflecting walls (a box without 'periodic') are not on the GPU force path "
+                         "(run this model with accusim itself)")
     s = System(dim, box=box, periodic=model.get("periodic", False))
     kinds = {"mass": "mass", "charge": "charge", "radius": "length", "diffusion": "diffusion"}
     for name, sp in model["species"].items():

@@ -1,15 +1,25 @@
-"""Simulation driver (accusim/simulation.py:14-127): sums the forces, steps the
-integrator, records the conserved quantities. Reactions are not on the force path;
-a `reactions` engine object with the reference's interface is still honoured.
+"""The host-side run loop over the GPU forces (accusim/simulation.py:14-127).
+
+`Simulation` keeps the reference's constructor, attributes (`system`, `forces`,
+`integrator`, `dt`, `rng`, `seed`, `time`, `steps`, `trajectory`, `force`, `pe`,
+`acc`) and methods (`potential`, `update_forces`, `record`, `run`, `clone`,
+`accuracy_report`), so that integrators, the accuracy report and user scripts written
+against accusim drive the device forces unchanged. Each `compute` returns a fresh float64
+array; `potential` adds them into one array of its own.
+
+What one record holds is defined in one place, `_observables`: time, kinetic and
+potential energy, internal energy, momentum, angular momentum, mass, charge, species
+counts and the thermostat's bath energy. Reactions are not part of this package's path:
+`reactions` must be empty, so nothing is released and the mass is the rest mass.
 """
 from __future__ import annotations
 
+import copy
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from .integrators import VelocityVerlet
-from .units import c as C_LIGHT
 from .units import si
 
 
@@ -32,8 +42,9 @@ class Trajectory:
 
     @property
     def total_energy(self) -> np.ndarray:
-        return (np.array(self.kinetic) + np.array(self.potential) + np.array(self.internal)
-                - np.array(self.released))
+        """Kinetic + potential + internal energy, less what reactions released."""
+        parts = [np.asarray(getattr(self, k), float) for k in ("kinetic", "potential", "internal", "released")]
+        return parts[0] + parts[1] + parts[2] - parts[3]
 
     def count(self, name) -> np.ndarray:
         return np.array([c.get(name, 0) for c in self.counts])
@@ -41,91 +52,99 @@ class Trajectory:
 
 class Simulation:
     def __init__(self, system, forces=(), reactions=None, integrator=None, dt=None, seed=None):
+        if reactions:
+            raise ValueError("reactions are not on the GPU force path (run such models with accusim itself)")
         self.system = system
         self.forces = list(forces)
-        self.integrator = integrator or VelocityVerlet()
-        self.dt = None if dt is None else si(dt, "s")
-        self.rng = np.random.default_rng(seed)
+        self.integrator = VelocityVerlet() if integrator is None else integrator
+        self.dt = si(dt, "s") if dt is not None else None
         self.seed = seed
-        self.reactions = reactions or None
+        self.rng = np.random.default_rng(seed)
+        self.reactions = None
         self.time = 0.0
         self.steps = 0
         self.trajectory = Trajectory()
+        self._initial = None
         self.update_forces()
         self._m0 = self._mass_energy()
-        self._initial = None
+
+    # ---- forces -------------------------------------------------------------------
 
     def potential(self):
-        s = self.system
-        f = np.zeros_like(s.pos)
-        u = 0.0
-        for force in self.forces:
-            fi, ui = force.compute(s)
-            f += fi
-            u += ui
-        return f, u
+        """(total force [N, dim], total potential energy) of every force on the current state."""
+        total = np.zeros_like(self.system.pos)
+        energy = 0.0
+        for f in self.forces:
+            Fi, Ui = f.compute(self.system)
+            total += Fi
+            energy += Ui
+        return total, energy
 
     def update_forces(self):
         self.force, self.pe = self.potential()
         m = self.system.masses
-        self.acc = self.force / m[:, None] if len(m) else self.force
+        self.acc = self.force / m[:, None] if m.size else self.force
 
     def _mass_energy(self):
-        released = self.reactions.released_rest if self.reactions else 0.0
-        return float(np.sum(self.system.masses)) + released / C_LIGHT ** 2
+        return float(np.sum(self.system.masses))
+
+    # ---- records ------------------------------------------------------------------
+
+    def _observables(self, with_state):
+        s = self.system
+        bath = getattr(self.integrator, "bath_energy", None)
+        obs = {
+            "t": self.time,
+            "kinetic": s.kinetic_energy(),
+            "potential": self.pe,
+            "internal": float(np.sum(getattr(s, "eint", 0.0))),
+            "released": 0.0,
+            "momentum": s.momentum(),
+            "angular_momentum": s.angular_momentum() if s.dim in (2, 3) else np.zeros(1),
+            "mass": self._mass_energy(),
+            "charge": float(np.sum(s.charges)),
+            "counts": s.counts(),
+            "bath": bath(s) if bath else 0.0,
+        }
+        if with_state:
+            obs.update(positions=s.pos.copy(), velocities=s.vel.copy(), species=s.sp.copy())
+        return obs
 
     def record(self, positions=True):
-        s, tr = self.system, self.trajectory
-        tr.t.append(self.time)
-        tr.kinetic.append(s.kinetic_energy())
-        tr.potential.append(self.pe)
-        tr.internal.append((self.reactions.internal if self.reactions else 0.0) + float(np.sum(getattr(s, "eint", 0.0))))
-        tr.released.append(self.reactions.released if self.reactions else 0.0)
-        tr.momentum.append(s.momentum())
-        tr.angular_momentum.append(s.angular_momentum() if s.dim in (2, 3) else np.zeros(1))
-        tr.mass.append(self._mass_energy())
-        tr.charge.append(float(np.sum(s.charges)))
-        tr.counts.append(s.counts())
-        bath = getattr(self.integrator, "bath_energy", None)
-        tr.bath.append(bath(s) if bath else 0.0)
-        if positions:
-            tr.positions.append(s.pos.copy())
-            tr.velocities.append(s.vel.copy())
-            tr.species.append(s.sp.copy())
+        for name, value in self._observables(positions).items():
+            getattr(self.trajectory, name).append(value)
+
+    # ---- stepping -----------------------------------------------------------------
 
     def run(self, duration=None, steps=None, record_every=1, positions=True):
         if self.dt is None:
             raise ValueError("set dt (or use accusim.advisor.suggest_dt)")
-        if steps is None:
-            steps = int(round(si(duration, "s") / self.dt))
+        n = int(round(si(duration, "s") / self.dt)) if steps is None else int(steps)
         if not self.trajectory.t:
+            # the first record also keeps the starting point for timestep_convergence
             if self._initial is None:
                 self._initial = self.clone()
             self.record(positions)
-        for k in range(1, steps + 1):
+        for k in range(1, n + 1):
             self.integrator.step(self, self.dt)
-            if self.reactions:
-                self.reactions.step(self, self.dt)
             self.time += self.dt
             self.steps += 1
-            if k % record_every == 0 or k == steps:
+            if k == n or k % record_every == 0:
                 self.record(positions)
         return self.trajectory
 
     def clone(self, dt=None):
-        import copy
-        sim = Simulation.__new__(Simulation)
-        sim.__dict__.update(self.__dict__)
-        sim.system = self.system.copy()
-        sim.forces = list(self.forces)
-        sim.integrator = copy.deepcopy(self.integrator)
-        sim.rng = np.random.default_rng(self.seed)
-        sim.reactions = copy.deepcopy(self.reactions)
-        sim.trajectory = Trajectory()
-        sim.dt = self.dt if dt is None else dt
-        sim.update_forces()
-        return sim
+        """A new simulation from the current state: its own copy of the system and of the
+        integrator's state, the same force objects (so their neighbour lists are shared),
+        a fresh generator from the seed and an empty trajectory; time and step count go on
+        from here."""
+        twin = Simulation(self.system.copy(), self.forces, integrator=copy.deepcopy(self.integrator),
+                          dt=self.dt if dt is None else dt, seed=self.seed)
+        twin.time, twin.steps = self.time, self.steps
+        twin._initial, twin._m0 = self._initial, self._m0
+        return twin
 
     def accuracy_report(self, convergence_duration=None):
         from .accuracy import AccuracyReport
         return AccuracyReport.build(self, convergence_duration)
+

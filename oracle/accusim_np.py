@@ -32,7 +32,11 @@ How each part is pinned (tests in tests/test_oracle_*.py):
                     energy error is second order in dt; a closed-form rigid
                     rotor (uniform rotation at the exact angular velocity)
   record / report   drift formulas on a trajectory with a known answer,
-                    golden vectors of a short argon run
+                    golden vectors of a short argon run; report_rows against the
+                    reference's own report rows written for small argon, Plummer
+                    Barnes-Hut, NaCl DSF and rigid-dimer runs (golden vectors)
+  timescales        closed forms: two-body softened orbit, sigma sqrt(m / eps),
+                    closest approach over speed; reference values (golden vectors)
   Langevin          the reference's thermostat test (tests/test_dynamics.py:80-95:
                     50 K -> 200 K within 8 %), the closed form at T = 0 (velocity
                     times c1 per step)
@@ -1071,6 +1075,127 @@ def force_approx_check(force, s):
     e = rms_rel_force_error(F_ap, F_ex)
     return _grade(e, 1e-3, 1e-2), e
 
+
+
+def timescales(sim, max_rows=1500):
+    """advisor.py:9-48 for the forces of this path (gravity, Barnes-Hut, LJ; the plain
+    Coulomb and bond rows need forces that are not on it): shortest times in seconds."""
+    s = sim.system
+    out = {}
+    if s.n >= 2:
+        sub = np.arange(s.n) if s.n <= max_rows else np.random.default_rng(0).choice(s.n, max_rows, replace=False)
+        d = s.pos[None, :, :] - s.pos[sub][:, None, :]
+        if s.periodic:
+            d -= s.box * np.round(d / s.box)
+        r = np.sqrt(np.einsum("ijk,ijk->ij", d, d))
+        r[np.arange(len(sub)), sub] = np.inf
+        m_all = s.masses
+        m = m_all[sub][:, None] * np.ones((1, s.n))
+        m_col = np.ones((len(sub), 1)) * m_all[None, :]
+        for f in sim.forces:
+            kind = type(f).__name__
+            if kind in ("Gravity", "BarnesHut"):
+                M = m + m_col
+                out["gravity (orbital)"] = float(np.min(np.sqrt((r ** 2 + f.eps2) ** 1.5 / (f.G * M))))
+            elif kind == "LennardJones":
+                eps_t, sig_t = f._tables(s)
+                for i, sp in enumerate(s.species):
+                    if eps_t[i, i] > 0:
+                        t = sig_t[i, i] * np.sqrt(sp.mass / eps_t[i, i])
+                        out["Lennard-Jones"] = min(out.get("Lennard-Jones", np.inf), float(t))
+        speed = np.max(np.linalg.norm(s.vel, axis=1))
+        if speed > 0 and any(not f.external for f in sim.forces):
+            eps2 = max([getattr(f, "eps2", 0.0) for f in sim.forces] or [0.0])
+            out["closest approach / speed"] = float(np.sqrt(np.min(r) ** 2 + eps2) / speed)
+    return out
+
+
+def report_rows(sim, dsf_ewald_max_n=2000):
+    """AccuracyReport.build without the timestep error, accuracy.py:59-189: the rows in the
+    reference's order as {name: (status, value or None)}. Reactions are absent, so the
+    reaction-timestep row never appears; `dsf_ewald_max_n` is the reference's 2000 unless
+    raised. The DSF row judges with this oracle's Ewald(k=f.k)."""
+    tr, s, integ = sim.trajectory, sim.system, sim.integrator
+    forces = sim.forces
+    rows = {"units": ("PASS", None)}
+    conservative = all(f.conservative for f in forces) and not integ.thermostat
+    external = any(f.external for f in forces)
+    walls = s.box is not None and not s.periodic
+    central = all(f.central for f in forces)
+    E = tr.total_energy
+    scale = max(abs(E[0]), np.max(np.abs(tr.kinetic)), np.max(np.abs(tr.potential)), 1e-300)
+    if conservative:
+        drift = float(np.max(np.abs(E - E[0])) / scale)
+        rows["energy"] = (_grade(drift, 1e-5, 1e-3), drift)
+    else:
+        rows["energy"] = ("n/a", None)
+    if hasattr(integ, "bath_energy") and all(f.conservative for f in forces) and len(tr.bath) == len(tr.t):
+        Ex = E + np.array(tr.bath)
+        dx = float(np.max(np.abs(Ex - Ex[0])) / scale)
+        rows["extended energy"] = (_grade(dx, 1e-5, 1e-3), dx)
+    P = np.array(tr.momentum)
+    pscale = max(float(np.sum(s.masses[:, None] * np.abs(s.vel))), 1e-300)
+    if not external and not walls and not integ.thermostat:
+        dp = float(np.max(np.linalg.norm(P - P[0], axis=1)) / pscale)
+        rows["momentum"] = (_grade(dp, 1e-9, 1e-6), dp)
+    else:
+        rows["momentum"] = ("n/a", None)
+    if central and not external and s.box is None and not integ.thermostat and s.dim in (2, 3):
+        L = np.array(tr.angular_momentum)
+        lscale = max(float(np.sum(np.linalg.norm(np.cross(s.pos, s.masses[:, None] * s.vel), axis=-1))), 1e-300)
+        dl = float(np.max(np.linalg.norm(L - L[0], axis=1)) / lscale)
+        rows["angular momentum"] = (_grade(dl, 1e-9, 1e-6), dl)
+    else:
+        rows["angular momentum"] = ("n/a", None)
+    M = np.array(tr.mass)
+    dm = float(np.max(np.abs(M - M[0])) / M[0]) if M[0] else 0.0
+    rows["mass-energy"] = (_grade(dm, 1e-12, 1e-9), dm)
+    Qc = np.array(tr.charge)
+    qscale = max(float(np.sum(np.abs(s.charges))), 1e-300)
+    dq = float(np.max(np.abs(Qc - Qc[0])) / qscale)
+    rows["charge"] = (_grade(dq, 1e-12, 1e-9), dq)
+    kind = type(integ).__name__
+    if kind in ("Langevin", "NoseHoover") and len(tr.t) >= 4:
+        half = len(tr.t) // 2
+        kin = np.asarray(tr.kinetic[half:])
+        counts = getattr(tr, "counts", None)
+        n = (np.array([sum(c.values()) for c in counts[half:]], dtype=float) if counts
+             else np.full(len(kin), float(s.n)))
+        dof = s.dim * np.maximum(n - 1, 1) if kind == "NoseHoover" else s.dim * np.maximum(n, 1)
+        T_meas = float(np.mean(2 * kin / (dof * K_B)))
+        dev = abs(T_meas - integ.T) / integ.T
+        tol = max(0.02, 3 * np.sqrt(2 / (s.dim * max(s.n, 1))))
+        rows["thermostat"] = ("PASS" if dev <= tol else ("WARN" if dev <= 0.1 else "FAIL"), dev)
+    if kind != "Brownian":
+        ts = timescales(sim)
+        if ts:
+            _, tmin = min(ts.items(), key=lambda kv: kv[1])
+            ratio = sim.dt / tmin
+            rows["timestep vs physics"] = (_grade(ratio, 0.02, 0.1), ratio)
+    approx = [f for f in forces if not getattr(f, "exact", True)]
+    if approx and s.n >= 2:
+        sub = np.random.default_rng(0).choice(s.n, size=min(s.n, 200), replace=False)
+        e = max(rms_rel_force_error(f.compute(s)[0][sub], direct_subset(f, s, sub)) for f in approx)
+        rows["force approximation"] = (_grade(e, 1e-3, 1e-2), e)
+    for f in forces:
+        if type(f).__name__ != "DampedShiftedCoulomb":
+            continue
+        q = s.charges
+        if not (s.periodic and s.dim == 3 and np.any(q)):
+            continue
+        if f.exclude or abs(q.sum()) > 1e-9 * np.sum(np.abs(q)) or s.n > dsf_ewald_max_n:
+            rows["DSF vs Ewald"] = ("n/a", None)
+            continue
+        e = rms_rel_force_error(f.compute(s)[0], Ewald(k=f.k).compute(s)[0])
+        rows["DSF vs Ewald"] = (_grade(e, 5e-2, 2e-1), e)
+    if s.periodic and any(type(f).__name__ == "Coulomb" for f in forces):
+        rows["electrostatics"] = ("WARN", None)
+    if hasattr(integ, "max_violation"):
+        v = integ.max_violation
+        rows["constraints"] = (_grade(v, 1e-9, 1e-6), v)
+    beta2 = float(np.max(np.sum(s.vel ** 2, axis=1)) / C_LIGHT ** 2) if s.n else 0.0
+    rows["relativity"] = (_grade(beta2, 1e-4, 1e-2), beta2)
+    return rows
 
 # ---------------------------------------------------------------- analysis.py:7-94
 
