@@ -40,6 +40,7 @@ SIGNATURES = {
     "accu_coulomb_forces": [P, P, P, I, I, D, D, D, D, P, I, P, I, I, P, P, P],
     "accu_ewald_recip": [P, P, I, P, D, P, I, D, I, P, P, P],
     "accu_gravity_direct": [P, P, I, P, I, D, D, P, I, I, P, P, P, P],
+    "accu_gravity_direct_noalloc": [P, P, I, P, I, D, D, P, I, I, P, P, P, P],
     "accu_bh_forces": [P, P, I, D, D, D, I, I, P, P, P, P],
 }
 _RESTYPES = {"accu_last_error": C.c_char_p}

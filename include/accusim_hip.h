@@ -83,6 +83,16 @@ int accu_ewald_recip(const double* pos, const double* q, int n, const double* bo
 int accu_gravity_direct(const double* pos, const double* mass, int n, const long long* targets, int ntgt, double G,
                         double eps2, const double* box_host, int periodic, int precision, const double* origin_host,
                         double* F, double* U, void* stream);
+/* The same computation without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Its energy and split partials live in a fixed
+ * 8 MiB device array of the library (one per GPU, held until the process ends); calls
+ * that need more return ACCU_EINVAL. The array keeps the last call's contents, so calls
+ * that overlap on one GPU (two streams, or two threads) race and must be serialised by
+ * the caller. Results equal accu_gravity_direct's bit for bit. */
+int accu_gravity_direct_noalloc(const double* pos, const double* mass, int n, const long long* targets, int ntgt,
+                                double G, double eps2, const double* box_host, int periodic, int precision,
+                                const double* origin_host, double* F, double* U, void* stream);
 
 /* ---- Barnes-Hut: tree.py:45-78 (tree built on the host from pos_host/m_host,
  * walked on the device). quadrupole != 0 adds quadrupole terms (extension). */
