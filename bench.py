@@ -202,6 +202,27 @@ def gravity_plummer(n=262144, seed=11):
     bh = dp.BarnesHut(softening=soft, theta=0.5, quadrupole=True)
     sec, (Fb, _) = timed(bh.compute)
     out["bh_theta05_quad_ms"] = 1e3 * sec
+    # the same tree build and walk on device-resident positions, timed with HIP events
+    pd = torch.from_numpy(s.pos).cuda()
+    md = torch.from_numpy(s.masses).cuda()
+    F = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    U = torch.empty(1, dtype=torch.float64, device="cuda")
+    bh.forces_device(pd, md, n, F, U)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    build_ms = walk_ms = 0.0
+    reps = 3
+    for _ in range(reps):
+        ev[0].record()
+        bh.tree.build(pd, md, n, bh.leaf_size, bh.quadrupole)
+        ev[1].record()
+        bh.tree.walk(bh.G, bh.eps2, bh.theta, F, U)
+        ev[2].record()
+        torch.cuda.synchronize()
+        build_ms += ev[0].elapsed_time(ev[1]) / reps
+        walk_ms += ev[1].elapsed_time(ev[2]) / reps
+    out["bh_device_build_ms"] = build_ms
+    out["bh_device_walk_ms"] = walk_ms
+    out["bh_tree"] = bh.tree.info()
     rng = np.random.default_rng(0)
     sub = rng.choice(n, 200, replace=False)
     Fd = dp.direct_subset(dp.Gravity(softening=soft), s, sub)

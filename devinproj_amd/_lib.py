@@ -41,7 +41,12 @@ SIGNATURES = {
     "accu_ewald_recip": [P, P, I, P, D, P, I, D, I, P, P, P],
     "accu_gravity_direct": [P, P, I, P, I, D, D, P, I, I, P, P, P, P],
     "accu_gravity_direct_noalloc": [P, P, I, P, I, D, D, P, I, I, P, P, P, P],
-    "accu_bh_forces": [P, P, I, D, D, D, I, I, P, P, P, P],
+    "accu_bh_create": [P],
+    "accu_bh_destroy": [P],
+    "accu_bh_build": [P, P, P, I, I, I, P],
+    "accu_bh_walk": [P, D, D, D, P, P, P],
+    "accu_bh_info": [P, P],
+    "accu_bh_export": [P, P, P, P, P, P, P, P, P, P],
 }
 _RESTYPES = {"accu_last_error": C.c_char_p}
 

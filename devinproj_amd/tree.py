@@ -1,7 +1,8 @@
-"""Barnes-Hut octree gravity (accusim/tree.py) with the walk on the GPU.
+"""Barnes-Hut octree gravity (accusim/tree.py) with the tree built and walked on the GPU.
 
-The tree is built exactly as tree.py:30-53 builds it; the walk accepts the same
-nodes (tree.py:67-70). `quadrupole=True` adds quadrupole moments to accepted
+The octree is the reference's (tree.py:30-53: root cube, octant order, leaf_size,
+depth < 48), built by the device from sorted octant-path keys; the walk accepts the
+same nodes (tree.py:67-70). `quadrupole=True` adds quadrupole moments to accepted
 nodes (an extension; the reference expansion is monopole only).
 """
 from __future__ import annotations
@@ -12,6 +13,61 @@ from . import _lib as L
 from .forces import Force
 from .units import G as G_CONST
 from .units import si
+
+
+class Octree:
+    """Device octree (accu_bh_* handle): build from device positions/masses, walk, export."""
+
+    def __init__(self):
+        self._h = None
+        self.n = 0
+
+    @property
+    def handle(self):
+        if self._h is None:
+            h = L.C.c_void_p()
+            L.call("accu_bh_create", L.C.byref(h))
+            self._h = h
+        return self._h
+
+    def build(self, pos_d, m_d, n, leaf_size=8, quadrupole=False):
+        L.call("accu_bh_build", self.handle, L.ptr(pos_d), L.ptr(m_d), int(n), int(leaf_size),
+               int(bool(quadrupole)), L.stream())
+        self.n = int(n)
+
+    def walk(self, G, eps2, theta, F_d, U_d=None):
+        L.call("accu_bh_walk", self.handle, float(G), float(eps2), float(theta), L.ptr(F_d), L.ptr(U_d), L.stream())
+
+    def info(self):
+        out = np.zeros(6, np.int64)
+        L.call("accu_bh_info", self.handle, L.ptr(out))
+        return {"n": int(out[0]), "nodes": int(out[1]), "leaves": int(out[2]), "max_depth": int(out[3]),
+                "deep": bool(out[4]), "leaf_size": int(out[5])}
+
+    def export(self, quadrupole=False):
+        """The tree in pre-order as numpy arrays (cube, com, next, leaf, start, count, perm[, quad])."""
+        th = L.torch()
+        nn = self.info()["nodes"]
+        f64 = dict(dtype=th.float64, device="cuda")
+        i32 = dict(dtype=th.int32, device="cuda")
+        t = {"cube": th.empty((nn, 4), **f64), "com": th.empty((nn, 4), **f64), "next": th.empty(nn, **i32),
+             "leaf": th.empty(nn, **i32), "start": th.empty(nn, **i32), "count": th.empty(nn, **i32),
+             "perm": th.empty(self.n, **i32)}
+        q = th.empty((nn, 6), **f64) if quadrupole else None
+        L.call("accu_bh_export", self.handle, L.ptr(t["cube"]), L.ptr(t["com"]), L.ptr(q), L.ptr(t["next"]),
+               L.ptr(t["leaf"]), L.ptr(t["start"]), L.ptr(t["count"]), L.ptr(t["perm"]), L.stream())
+        out = {k: v.cpu().numpy() for k, v in t.items()}
+        if q is not None:
+            out["quad"] = q.cpu().numpy()
+        return out
+
+    def __getstate__(self):
+        return {"n": 0, "_h": None}
+
+    def __del__(self):
+        if self._h is not None and L._lib is not None:
+            L._lib.accu_bh_destroy(self._h)
+            self._h = None
 
 
 class BarnesHut(Force):
@@ -26,7 +82,13 @@ class BarnesHut(Force):
         self.leaf_size = leaf_size
         self.quadrupole = quadrupole
         self.nodes = 0
+        self.tree = Octree()
         self._buf = L.Buffers()
+
+    def forces_device(self, pos_d, m_d, n, F_d, U_d=None):
+        """Build the tree over device positions/masses and walk it into device F (and U)."""
+        self.tree.build(pos_d, m_d, n, self.leaf_size, self.quadrupole)
+        self.tree.walk(self.G, self.eps2, self.theta, F_d, U_d)
 
     def compute(self, s):
         if s.n < 2:
@@ -34,14 +96,12 @@ class BarnesHut(Force):
         if s.dim != 3:
             raise ValueError("GPU Barnes-Hut needs dim == 3")
         th = L.torch()
-        pos = np.ascontiguousarray(s.pos, float)
-        m = np.ascontiguousarray(s.masses, float)
+        pos = self._buf.upload("pos", np.asarray(s.pos, float), th.float64)
+        m = self._buf.upload("m", np.asarray(s.masses, float), th.float64)
         F = self._buf.get("F", (s.n, 3), th.float64)
         U = self._buf.get("U", (1,), th.float64)
-        nn = L.C.c_longlong(0)
-        L.call("accu_bh_forces", L.ptr(pos), L.ptr(m), s.n, float(self.G), float(self.eps2), float(self.theta),
-               int(self.leaf_size), int(bool(self.quadrupole)), L.ptr(F), L.ptr(U), L.C.byref(nn), L.stream())
-        self.nodes = nn.value
+        self.forces_device(pos, m, s.n, F, U)
+        self.nodes = self.tree.info()["nodes"]
         return F.cpu().numpy(), float(U.cpu().numpy()[0])
 
 

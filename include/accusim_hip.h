@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define ACCU_ABI_VERSION 1
+#define ACCU_ABI_VERSION 2
 
 enum { ACCU_FP64 = 0, ACCU_FP32_PAIR = 1 };
 
@@ -94,10 +94,28 @@ int accu_gravity_direct_noalloc(const double* pos, const double* mass, int n, co
                                 double G, double eps2, const double* box_host, int periodic, int precision,
                                 const double* origin_host, double* F, double* U, void* stream);
 
-/* ---- Barnes-Hut: tree.py:45-78 (tree built on the host from pos_host/m_host,
- * walked on the device). quadrupole != 0 adds quadrupole terms (extension). */
-int accu_bh_forces(const double* pos_host, const double* m_host, int n, double G, double eps2, double theta,
-                   int leaf_size, int quadrupole, double* F, double* U, long long* nnodes_out, void* stream);
+/* ---- Barnes-Hut: tree.py:19-78 (BarnesHut._build and compute). The octree is
+ * built and walked on the device; the handle keeps the tree and its buffers. */
+typedef struct accu_bh accu_bh;
+int accu_bh_create(accu_bh** out);
+int accu_bh_destroy(accu_bh* h);
+/* tree.py:30-53: the reference's octree (root cube, octant order, leaf_size, depth
+ * < 48) over device pos [n][3] and m [n]; quadrupole != 0 also keeps traceless
+ * quadrupole moments (an extension). Synchronises once, to size the node arrays. */
+int accu_bh_build(accu_bh* h, const double* pos, const double* m, int n, int leaf_size, int quadrupole,
+                  void* stream);
+/* tree.py:54-78 over the last built tree: F [n][3] (user order) = m * acc and U =
+ * 1/2 sum m phi (U may be NULL); quadrupole terms when the tree has them. */
+int accu_bh_walk(accu_bh* h, double G, double eps2, double theta, double* F, double* U, void* stream);
+/* out_host[6] = n, nodes, leaves, max depth, deep (1 if more than 21 levels were
+ * sorted), leaf_size; synchronises */
+int accu_bh_info(accu_bh* h, long long* out_host);
+/* copies of the tree in pre-order (device buffers, any may be NULL): cube [nodes][4]
+ * (center, half), com [nodes][4] (center of mass, mass), quad [nodes][6] (xx, yy, zz,
+ * xy, xz, yz), next / leaf / start / count int32 [nodes] (first node after the
+ * subtree, leaf flag, particle run in tree order), perm int32 [n] (tree order) */
+int accu_bh_export(accu_bh* h, double* cube, double* com, double* quad, int* next, int* leaf, int* start,
+                   int* count, int* perm, void* stream);
 
 /* ---- device-resident MD: Simulation.run (simulation.py:95-112) driving
  * VelocityVerlet / Yoshida4 (integrators.py:19-47) over LJ (forces.py:119-142) and/or

@@ -636,6 +636,42 @@ class BarnesHut(Force):
         return m[:, None] * acc, 0.5 * float(np.sum(m * phi))
 
 
+def bh_tree_preorder(pos, m, leaf_size=8):
+    """The octree of tree.py:51-53 (root cube) and 30-43 (_build), flattened depth-first in
+    pre-order with children in octant order 0..7, as flat arrays:
+      center [nn, 3], half [nn], depth [nn], leaf [nn] (bool), count [nn], mass [nn],
+      com [nn, 3], next [nn] (index of the first node after the subtree, nn at the end),
+      members: list of sorted particle-index arrays, one per node.
+    Pinned by BarnesHut itself: the walk over this tree is the one pinned by theta = 0 and
+    the golden theta = 0.5 / 0.3 vectors."""
+    pos = np.asarray(pos, float)
+    m = np.asarray(m, float)
+    lo, hi = pos.min(0), pos.max(0)
+    half = max(float(np.max(hi - lo)) / 2 * (1 + 1e-9), 1e-300)
+    root = BarnesHut(leaf_size=leaf_size)._build(pos, m, np.arange(len(pos)), (lo + hi) / 2, half)
+    rows = []
+
+    def visit(node, depth):
+        k = len(rows)
+        rows.append([node, depth, None])
+        for c in node.children:
+            visit(c, depth + 1)
+        rows[k][2] = len(rows)
+
+    visit(root, 0)
+    return {
+        "center": np.array([r[0].center for r in rows], float),
+        "half": np.array([r[0].half for r in rows], float),
+        "depth": np.array([r[1] for r in rows], np.int64),
+        "leaf": np.array([not r[0].children for r in rows], bool),
+        "count": np.array([len(r[0].idx) for r in rows], np.int64),
+        "mass": np.array([r[0].mass for r in rows], float),
+        "com": np.array([r[0].com for r in rows], float),
+        "next": np.array([r[2] for r in rows], np.int64),
+        "members": [np.sort(r[0].idx) for r in rows],
+    }
+
+
 def direct_subset(force, s, sub):
     """tree.py:81-87: exact gravitational force on particles `sub`."""
     d = s.pos[None, :, :] - s.pos[sub][:, None, :]
