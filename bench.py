@@ -17,8 +17,10 @@ roofline: the fused force + integrate kernel. Algorithmic bytes per launch
   stream over the timed region; peak 8 TB/s (HBM3E). traffic: the HBM bytes per
   launch that rocprofv3's FETCH_SIZE / WRITE_SIZE counters measure (calibrated on a
   plain copy).
-cpu_baseline: the NumPy float64 oracle (one thread) on a 131,072-atom sample of the
-  same density (5 VV steps after one list build), scaled to 1,048,576 atoms by N.
+cpu_baseline: the NumPy float64 oracle on the same 1,048,576-atom workload, one core
+  (a child process pinned to one CPU, with OMP/OpenBLAS/MKL threads set to 1 before
+  numpy loads), started first and run beside the GPU sections: one list build with the
+  first force evaluation (timed on its own), then 2 timed VV steps.
 """
 from __future__ import annotations
 
@@ -120,9 +122,11 @@ def brick_nacl(precision, steps, warmup):
     return out
 
 
-def drift_long(s, precision, steps=10_000):
+def drift_long(s, precision, steps=10_000, transient=1_000):
     """Energy drift (accuracy.py:72-76 formula) over `steps` recorded VV steps of the 1M-atom
-    workload, state resident in HBM."""
+    workload, state resident in HBM; also the drift after the start-up transient (from
+    step `transient` on, same normalisation) and the energy series for the mixed-minus-fp64
+    comparison."""
     import devinproj_amd as dp
     from devinproj_amd import batch
     from devinproj_amd import workloads as W
@@ -134,10 +138,17 @@ def drift_long(s, precision, steps=10_000):
     eng.run(steps, record_every=1)
     t1 = time.perf_counter()
     eng.fetch(state=False)
+    lg = eng.logs[0]
+    E = lg[:, 0] + lg[:, 1]
+    scale = max(abs(E[0]), np.max(np.abs(lg[:, 1])), np.max(np.abs(lg[:, 0])), 1e-300)
+    late = E[transient:]
+    slope = np.polyfit(np.arange(len(late), dtype=float), late, 1)[0]
     out = {"steps": steps, "energy_drift": eng.energy_drift(), "rebuilds": int(eng.rebuilds[0]),
-           "steps_per_s": steps / (t1 - t0)}
+           "steps_per_s": steps / (t1 - t0),
+           "energy_drift_after_step_%d" % transient: float(np.max(np.abs(late - late[0])) / scale),
+           "secular_drift_per_10k_steps": float(abs(slope) * 10_000 / scale)}
     eng.close()
-    return out
+    return out, E, scale
 
 
 def richardson_batch(coarse_steps=2000, dt=0.005, precision="mixed"):
@@ -228,11 +239,20 @@ def gravity_plummer(n=262144, seed=11):
     Fd = dp.direct_subset(dp.Gravity(softening=soft), s, sub)
     out["bh_rms_rel_force_error"] = rms_rel_error(Fb[sub], Fd)
     t_cross = float(np.sqrt(a ** 3 / (dp.G * M)))
-    sim = dp.Simulation(s, [dp.Gravity(softening=soft, precision="mixed")], integrator=dp.Yoshida4(),
-                        dt=1e-3 * t_cross)
-    t0 = time.perf_counter()
-    sim.run(steps=2, positions=False)
-    out["yoshida4_direct_mixed_s_per_step"] = (time.perf_counter() - t0) / 2
+    # device-resident Yoshida4 (3 fused force + kick + drift launches per step, one hipGraph
+    # per block of steps), direct mixed and fp64, and with the Barnes-Hut tree rebuilt per substep
+    for name, force, graph, steps in (("direct_mixed", dp.Gravity(softening=soft, precision="mixed"), True, 4),
+                                      ("direct_fp64", dp.Gravity(softening=soft), True, 2),
+                                      ("bh_theta05_quad", dp.BarnesHut(softening=soft, theta=0.5, quadrupole=True),
+                                       False, 4)):
+        eng = dp.NBodyEngine(s.copy(), force, dp.Yoshida4(), 1e-3 * t_cross, log_cap=64)
+        eng.run(1, use_graph=graph)
+        t0 = time.perf_counter()
+        eng.run(steps, use_graph=graph)
+        out[f"yoshida4_{name}_s_per_step"] = (time.perf_counter() - t0) / steps
+        eng.fetch(state=False)
+        out[f"yoshida4_{name}_energy_drift"] = eng.energy_drift()
+        eng.close()
     return out
 
 
@@ -313,21 +333,56 @@ def pairs_within(s, rc):
     return len(i)
 
 
-def cpu_baseline(n_cells=32, steps=5):
-    """Oracle (NumPy float64, one thread) on a 4*n_cells^3-atom sample at the same density."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def cpu_child(steps=2):
+    """The oracle on the 1,048,576-atom workload in this (child) process: list build plus
+    first forces, then `steps` timed VV steps. Prints one JSON line."""
     from oracle import accusim_np as A
     from oracle import builders as B
-    s = B.argon(n_cells, seed=2, jitter=0.01)
+    s = B.argon(64, seed=2, jitter=0.01)
+    t0 = time.perf_counter()
     sim = A.Simulation(s, [A.LennardJones(B.argon_params(), cutoff=2.5, skin=0.3)], integrator=A.VelocityVerlet(),
                        dt=DT)
-    t0 = time.perf_counter()
-    sim.run(steps=steps, positions=False)
     t1 = time.perf_counter()
-    sps = steps / (t1 - t0)
-    return {"value": sps * s.n / 1048576, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{s.n}-atom FCC argon at the same density, {steps} VV steps after one list build, "
-                      f"{sps:.3f} steps/s measured, scaled by N to 1,048,576 atoms"}
+    sim.run(steps=steps, positions=False)
+    t2 = time.perf_counter()
+    print(json.dumps({"atoms": s.n, "init_s": t1 - t0, "steps": steps, "steps_s": t2 - t1}), flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline_start():
+    """Start the oracle child on one CPU of this process's set, thread pools at 1."""
+    import subprocess
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    cpus = sorted(os.sched_getaffinity(0))
+    cpu = cpus[-1]
+    p = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", str(cpu)], env=env,
+                         stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    return p, len(cpus)
+
+
+def cpu_baseline_finish(p, host_cpus, timeout=900):
+    out, err = p.communicate(timeout=timeout)
+    if p.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({p.returncode}): {err[-2000:]}")
+    r = json.loads(out.strip().splitlines()[-1])
+    return {"value": r["steps"] / r["steps_s"], "unit": "steps/s", "cores": 1, "kind": "port",
+            "host_cpus": host_cpus, "cpu_model": cpu_model(),
+            "list_build_and_first_forces_s": r["init_s"],
+            "sample": f"the {r['atoms']}-atom argon workload itself: one cell-list build with the first force "
+                      f"evaluation ({r['init_s']:.1f} s), then {r['steps']} VV steps timed "
+                      f"({r['steps_s']:.1f} s); one core, thread pools at 1"}
 
 
 def main():
@@ -346,10 +401,19 @@ def main():
     ap.add_argument("--no-rattle", action="store_true", help="skip the rigid-dimer Rattle section")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--cpu-child", type=int, default=None, metavar="CPU", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child is not None:
+        os.sched_setaffinity(0, {args.cpu_child})
+        cpu_child()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
+    cpu_proc = None
+    if not args.no_cpu:
+        cpu_proc, host_cpus = cpu_baseline_start()
+        log("cpu baseline (oracle) started in a child process")
     import torch
 
     from devinproj_amd import _lib
@@ -376,7 +440,11 @@ def main():
     log(f"learned: {learned}")
     drift = None
     if not args.no_drift:
-        drift = {p: drift_long(s.copy(), p) for p in ("mixed", "fp64")}
+        runs = {p: drift_long(s.copy(), p) for p in ("mixed", "fp64")}
+        drift = {p: r[0] for p, r in runs.items()}
+        (Em, scale), (Ed, _) = (runs["mixed"][1:], runs["fp64"][1:])
+        k = min(len(Em), len(Ed))
+        drift["mixed_minus_fp64_energy_max_rel"] = float(np.max(np.abs(Em[:k] - Ed[:k])) / scale)
         log(f"drift: {drift}")
     rich = None if args.no_richardson else richardson_batch()
     log(f"richardson: {rich}")
@@ -387,9 +455,9 @@ def main():
     alg_bytes = n * 148 + 4 * (mixed["list_entries"] / 2)
     achieved = alg_bytes / (mixed["kernel_ms"] * 1e-3) / 1e9
     cpu = None
-    if not args.no_cpu:
-        log("cpu baseline (oracle)")
-        cpu = cpu_baseline()
+    if cpu_proc is not None:
+        log("waiting for the cpu baseline (oracle)")
+        cpu = cpu_baseline_finish(cpu_proc, host_cpus)
         log(f"cpu: {cpu}")
     line = {
         "metric": "MD steps/s at 1M-atom LJ & 65k-ion DSF: mixed/learned vs plain FP64, same GPU",

@@ -12,6 +12,7 @@ from .forces import (DampedShiftedCoulomb, Ewald, Force, Gravity, LennardJones, 
 from .integrators import Brownian, Integrator, Langevin, NoseHoover, Rattle, VelocityVerlet, Yoshida4
 from . import autotune, batch, workloads
 from .learned import LearnedDSF, PairCorrection, train_correction
+from .nbody import NBodyEngine
 from .simulation import Simulation, Trajectory
 from .system import Species, System
 from .tree import BarnesHut, direct_subset
@@ -19,7 +20,7 @@ from .units import G, k_B, k_e, si
 
 __all__ = [
     "AccuracyReport", "BarnesHut", "BrickEngine", "Brownian", "Check", "DampedShiftedCoulomb", "Ewald", "Force", "G",
-    "Gravity", "Integrator", "Langevin", "LearnedDSF", "LennardJones", "MDEngine", "NeighbourList", "NoseHoover",
+    "Gravity", "Integrator", "Langevin", "LearnedDSF", "LennardJones", "MDEngine", "NBodyEngine", "NeighbourList", "NoseHoover",
     "PairCorrection", "Rattle", "Simulation", "Species", "System",
     "Trajectory", "VelocityVerlet", "train_correction",
     "VerletList", "Yoshida4", "cell_list_pairs", "direct_subset", "k_B", "k_e", "kvector_columns", "si",

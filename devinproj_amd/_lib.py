@@ -47,6 +47,15 @@ SIGNATURES = {
     "accu_bh_walk": [P, D, D, D, P, P, P],
     "accu_bh_info": [P, P],
     "accu_bh_export": [P, P, P, P, P, P, P, P, P, P],
+    "accu_nbody_create": [P, I, P, P, P, P, I],
+    "accu_nbody_destroy": [P],
+    "accu_nbody_set_direct": [P, D, D, I],
+    "accu_nbody_set_bh": [P, D, D, D, I, I],
+    "accu_nbody_init": [P, I],
+    "accu_nbody_run": [P, I, D, P, I, I, I],
+    "accu_nbody_sync": [P],
+    "accu_nbody_get": [P, P, P, P, P, I, P],
+    "accu_nbody_stream": [P, P],
 }
 _RESTYPES = {"accu_last_error": C.c_char_p}
 

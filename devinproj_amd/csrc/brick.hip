@@ -55,6 +55,13 @@ struct BSys {
   double rc_list2;
   double half_skin2;
   double dt;
+  // mixed mode: the fixed-point position copy (offsets from each atom's cell corner in
+  // quanta of unit * 2^-fx_k, `unit` being the staging unit of the fused launch)
+  double hc[3];    // cell edge L / nc
+  double fx_q;     // 2^fx_k / unit: metres -> quanta
+  float fx_s;      // 2^-fx_k: quanta -> staging units of the fused launch (exact)
+  float fx_sm;     // unit * 2^-fx_k: quanta -> metres (the list launch)
+  int fx_k;
 };
 
 struct BBrick {
@@ -236,7 +243,62 @@ struct BState {
   double4* ref[2];  // positions at the last list build
   int* perm[2];     // user index of each slot
   int* cell[2];     // cell id of each slot
+  // mixed mode only (else null): the positions as fixed-point offsets from the corner of
+  // each atom's cell, species in w; posf[0] pairs with posA and posf[1] with posB (the
+  // position parity, not the state parity)
+  int4* posf[2];
+  const double4* ccorner;  // corner of every cell in the box frame (both modes)
 };
+
+// Fixed-point copy of position x of an atom of the cell with corner cc: per axis the offset
+// from the corner at the periodic image nearest the cell's centre, in quanta of
+// unit * 2^-fx_k (|offset| stays within a cell and half a skin, far inside int32).
+__device__ __forceinline__ int4 fix_pos(const BSys& s, const double4& x, const double4& cc) {
+  const double o[3] = {x.x - cc.x, x.y - cc.y, x.z - cc.z};
+  int q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double e = fma(-s.L[d], rint((o[d] - 0.5 * s.hc[d]) * s.inv[d]), o[d]);
+    q[d] = (int)rint(e * s.fx_q);
+  }
+  return make_int4(q[0], q[1], q[2], (int)x.w);
+}
+
+// Mixed mode: stage the region from the fixed-point copy. Region cell r (an image of cell
+// gid shifted by s L) has its corner at B = corner(gid) + s L - org relative to the brick
+// centre, so an atom's staged coordinate is (B + offset) in quanta: an exact integer sum,
+// then one rounding to float and an exact power-of-two scale (`scale` = 2^-fx_k), which is
+// the value the float64 staging rounds to, up to one quantum. 16 bytes per atom read
+// instead of 32, and no float64 image arithmetic per atom.
+template <bool PK3>
+__device__ void stage_region_fix(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
+                                 const int* __restrict__ start, const int* rs, const int4* __restrict__ posf,
+                                 const double4* __restrict__ ccorner, float4* lds, float scale) {
+  const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  const int G = max(1, (int)blockDim.x / R);  // threads per region cell
+  const int per = (int)blockDim.x / G;         // region cells per pass
+  const int sub = threadIdx.x % G;
+  for (int r = threadIdx.x / G; r < R && threadIdx.x < per * G; r += per) {
+    const int q0 = rs[r], q1 = rs[r + 1];
+    const BRegion rg = reg[b.reg_off + r];
+    const int base = start[rg.gid] - q0;
+    const double4 cc = ccorner[rg.gid];
+    const double c3[3] = {cc.x, cc.y, cc.z};
+    const int sh[3] = {rg.shift % 3 - 1, (rg.shift / 3) % 3 - 1, rg.shift / 9 - 1};
+    int B[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) B[d] = (int)rint((c3[d] + sh[d] * s.L[d] - b.org[d]) * s.fx_q);
+    for (int q = q0 + sub; q < q1; q += G) {
+      const int4 o = posf[base + q];
+      float4 v;
+      v.x = (float)(B[0] + o.x) * scale;
+      v.y = (float)(B[1] + o.y) * scale;
+      v.z = (float)(B[2] + o.z) * scale;
+      v.w = (float)o.w;
+      st_slot<PK3>(lds, q, v);
+    }
+  }
+}
 
 struct RB {
   const BSys* sys;
@@ -388,8 +450,13 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
     if (threadIdx.x == 0) atomicMax(&p.ints[1], total + 1);
     return;
   }
-  // PK3: 12-byte slots (the list test needs no species), so that more bricks share a CU's LDS
-  stage_region<float, PK3>(b, s, p.reg, p.start, rs, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
+  // PK3: 12-byte slots (the list test needs no species), so that more bricks share a CU's LDS;
+  // mixed mode stages from the fixed-point copy (in metres: the margins below cover the
+  // one extra rounding of the scale)
+  if (p.st.posf[0])
+    stage_region_fix<PK3>(b, s, p.reg, p.start, rs, p.st.posf[p.par[b.sys]], p.st.ccorner, lds32, s.fx_sm);
+  else
+    stage_region<float, PK3>(b, s, p.reg, p.start, rs, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
   __syncthreads();
   const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
@@ -582,6 +649,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       const int pc = rb_par(p, si), sc = p.spar[si];
       const double4 x = (pc ? p.posB : p.posA)[v];
       (pc ? p.posA : p.posB)[t] = x;
+      if (p.st.posf[0]) p.st.posf[pc ^ 1][t] = fix_pos(p.sys[si], x, p.st.ccorner[c]);
       p.st.ref[sc ^ 1][t] = x;
       p.st.vel[sc ^ 1][t] = p.st.vel[sc][v];
       const int u = p.st.perm[sc][v];
@@ -707,15 +775,6 @@ __global__ void k_b_fill(int* a, int n, int v) {
 
 // ------------------------------------------------------------------ fused step
 
-__device__ __forceinline__ double np_mod2(double x, double L) {
-  double r = fmod(x, L);
-  if (r != 0.0) {
-    if ((r < 0.0) != (L < 0.0)) r += L;
-  } else {
-    r = copysign(0.0, L);
-  }
-  return r;
-}
 
 // LJ pair of the engine (forces.py:127-141; full list, so half the energy per visit).
 // The cutoff is a select on 1/r2 rather than a branch, so the eight pairs of a group run
@@ -724,13 +783,14 @@ __device__ __forceinline__ double np_mod2(double x, double L) {
 // exact zero for pairs outside and nothing overflows for them. With x = sig2 w,
 // sr6 = x^3 and b = sr6^2 - sr6:
 //   f += (sr6^2 + b) w d            (times -24 eps),
-//   u += b, n += 1 for a pair inside (the energy, 2 eps (u - shift n), is formed per atom).
-// EPS: several species, so the pair's eps multiplies f, u and n (n then sums 2 eps over
-// the pairs inside); with one species the factors are applied once per atom.
+//   u += b, ni += 1 for a pair inside (the energy, 2 eps (u - shift ni), is formed per atom;
+//   the count is an integer, one add with carry per pair).
+// EPS: several species, so the pair's eps multiplies f and u, and n sums 2 eps over the
+// pairs inside; with one species the factors are applied once per atom.
 // UNIT_SIG: the staged coordinates are in units of sigma, so x = w.
 template <typename P, bool EN, bool EPS, bool UNIT_SIG>
 __device__ __forceinline__ void lj_engine(P dx, P dy, P dz, P r2, P e2, P m24, P sig2, P rc2, P& fx, P& fy, P& fz,
-                                          P& u, P& n) {
+                                          P& u, P& n, unsigned& ni) {
   P ir2;
   if constexpr (sizeof(P) == 8) ir2 = rcp64(r2);
   else ir2 = __builtin_amdgcn_rcpf(r2);
@@ -751,15 +811,15 @@ __device__ __forceinline__ void lj_engine(P dx, P dy, P dz, P r2, P e2, P m24, P
       n += e;
     } else {
       u += b;
-      n += in ? P(1) : P(0);
+      ni += in ? 1u : 0u;
     }
   }
 }
 
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN>
 __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P qi, const typename P4<P>::T& xi,
-                                       const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& n, P& cx, P& cy,
-                                       P& cz, P& cu) {
+                                       const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& n,
+                                       unsigned& ni, P& cx, P& cy, P& cz, P& cu) {
   const P dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
   const int tj = ONE_SP ? 0 : (int)xj.w;
   // list membership is decided at the build, so r^2 can use fused multiply-adds here
@@ -767,7 +827,7 @@ __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P q
   if constexpr (LJ_ON) {
     const int pj = row + tj;
     lj_engine<P, EN, !ONE_SP, ONE_SP && !COUL_ON>(dx, dy, dz, r2, ljt[pj], ljt[64 + pj], ljt[128 + pj],
-                                                  ljt[192 + pj], fx, fy, fz, u, n);
+                                                  ljt[192 + pj], fx, fy, fz, u, n, ni);
   }
   if constexpr (COUL_ON) {
     if constexpr (sizeof(P) == 8)
@@ -856,7 +916,10 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
     if (f < snap_cap) frame = snap + (size_t)f * N * 6;
   }
   if (ok) {
-    stage_region<P, PK3>(b, s, reg, start, rs, pos, lds, 1.0 / unit);
+    if constexpr (sizeof(P) == 4)
+      stage_region_fix<PK3>(b, s, reg, start, rs, sta.posf[par[si]], sta.ccorner, lds, s.fx_s);
+    else
+      stage_region<P, PK3>(b, s, reg, start, rs, pos, lds, 1.0 / unit);
     if (threadIdx.x == 0) {  // far sentinel: the padding entries of the lists point here
       T4 far;
       far.x = (P)1e15;
@@ -890,6 +953,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       const P qi = (P)s.charge[ti];
       const int ng = (nn[a] + 7) >> 3;
       P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
+      unsigned ni = 0;  // pairs inside the cutoff (one species)
       // groups of 8 neighbours; the next group's indices load while this one computes
       uint4 cur = make_uint4(0u, 0u, 0u, 0u);
       if (sub < ng) cur = nbr[(size_t)sub * stride + a];
@@ -902,10 +966,10 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           const T4 x0 = ld_slot<PK3>(lds, w[2 * h2] & 0xffffu), x1 = ld_slot<PK3>(lds, w[2 * h2] >> 16);
           const T4 x2 = ld_slot<PK3>(lds, w[2 * h2 + 1] & 0xffffu);
           const T4 x3 = ld_slot<PK3>(lds, w[2 * h2 + 1] >> 16);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, n, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, n, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, n, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, n, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
+          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
         }
         if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 8 neighbours
           Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
@@ -914,11 +978,12 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
         cur = nxt;
       }
       Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
+      Nin += (double)ni;
       }
       if (split) {
 #pragma unroll
         for (int o = 1; o < kSplit; o <<= 1) {
-          if (o >= L) break;
+          if (o >= L) continue;  // L is the same for the whole block
           Fx += __shfl_xor(Fx, o, 64);
           Fy += __shfl_xor(Fy, o, 64);
           Fz += __shfl_xor(Fz, o, 64);
@@ -984,7 +1049,10 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           frame[3 * ((size_t)N + u) + 2] = v.z;
         }
       }
-      Fst[a] = make_double4(Fx, Fy, Fz, 0.0);
+      // the force is kept only where the next kick runs in another launch (no drift here:
+      // the last step of a run, the thermostatted integrators); a drifting brick has
+      // already used it for both half kicks
+      if (!do_drift) Fst[a] = make_double4(Fx, Fy, Fz, 0.0);
       if (do_drift) {
         double4 x = pos[a];
         {
@@ -1000,6 +1068,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
         x.y = np_mod2(x.y, s.L[1]);
         x.z = np_mod2(x.z, s.L[2]);
         pos_next[a] = x;
+        if constexpr (sizeof(P) == 4) sta.posf[par[si] ^ 1][a] = fix_pos(s, x, sta.ccorner[cell[a]]);
         const double4 r = ref[a];
         const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
         const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
@@ -1159,6 +1228,7 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
       x.y = np_mod2(x.y, s.L[1]);
       x.z = np_mod2(x.z, s.L[2]);
       pos_next[a] = x;
+      if (sta.posf[0]) sta.posf[par[si] ^ 1][a] = fix_pos(s, x, sta.ccorner[sta.cell[sp][a]]);
       const double4 r = ref[a];
       const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
       const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
@@ -1475,6 +1545,7 @@ __device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restric
     v.y = v.y + D(5 * j + 1) / s.dt;
     v.z = v.z + D(5 * j + 2) / s.dt;
     nw[t] = x;
+    if (sta.posf[0]) sta.posf[par[si] ^ 1][t] = fix_pos(s, x, sta.ccorner[sta.cell[sp][t]]);
     vel[t] = v;
     D(5 * j) = x.x;  // the displacement is applied: keep the final position for the check below
     D(5 * j + 1) = x.y;
@@ -1699,7 +1770,8 @@ struct BMD {
     void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, spar, sta.vel[0],
                   sta.vel[1], sta.F[0], sta.F[1], sta.ref[0], sta.ref[1], sta.perm[0], sta.perm[1], sta.cell[0],
                   sta.cell[1], count, start, tmp, slot, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
-                  bar, bsum, tphase, snap, snap_count, snap_rec, ndrift, nh};
+                  bar, bsum, tphase, snap, snap_count, snap_rec, ndrift, nh, sta.posf[0], sta.posf[1],
+                  (void*)sta.ccorner};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     for (void* p : cbuf) (void)hipFree(p);
@@ -1930,6 +2002,7 @@ static int build_tables(BMD& m) {
   std::vector<BRegion> hr;
   std::vector<int> cellmap, cellsys;
   std::vector<std::array<int, 3>> corner;  // low corner (cell coords) of every brick
+  std::vector<double4> hcc;                // corner of every cell (box frame), by global cell id
   m.brick_off.assign(S + 1, 0);
   m.atom_off.assign(S + 1, 0);
   int cell_off = 0;
@@ -1983,6 +2056,25 @@ static int build_tables(BMD& m) {
     s.rc_list2 = sp.rc_list * sp.rc_list;
     s.half_skin2 = sp.skin > 0 ? (0.5 * sp.skin) * (0.5 * sp.skin) : -1.0;
     s.dt = sp.dt;
+    // fixed-point copy: the fused launch's staging unit (sigma for one-species LJ without
+    // Coulomb, as the kernel decides it, else metres) and the largest power of two of
+    // quanta per unit that keeps a staged coordinate (half a brick plus three cells)
+    // below 2^30 quanta
+    {
+      const bool unit_sig = m.any_lj && m.one_sp && !m.any_coul && sp.has_lj && sp.eps[0] > 0.0 && sp.rc2[0] > 0.0;
+      const double unit = unit_sig ? std::sqrt(sp.sig2[0]) : 1.0;
+      double ext = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        s.hc[d] = sp.box[d] / s.nc[d];
+        ext = std::max(ext, (0.5 * sp.bdim[d] + 3.0) * s.hc[d] / unit);
+      }
+      const int k = (int)std::floor(std::log2((double)(1 << 30) / ext));
+      ARG_CHECK(k > -100 && k < 100, "system %d: no fixed-point scale for cells of %.3g", si, s.hc[0]);
+      s.fx_k = k;
+      s.fx_q = std::ldexp(1.0, k) / unit;
+      s.fx_s = std::ldexp(1.0f, -k);
+      s.fx_sm = (float)(unit * std::ldexp(1.0, -k));
+    }
     // bricks and brick-major cell numbering
     const int nbx[3] = {(s.nc[0] + sp.bdim[0] - 1) / sp.bdim[0], (s.nc[1] + sp.bdim[1] - 1) / sp.bdim[1],
                         (s.nc[2] + sp.bdim[2] - 1) / sp.bdim[2]};
@@ -2014,6 +2106,12 @@ static int build_tables(BMD& m) {
           hb.push_back(b);
           corner.push_back({lo[0], lo[1], lo[2]});
         }
+    hcc.resize(gid);
+    for (int cx = 0; cx < s.nc[0]; ++cx)
+      for (int cy = 0; cy < s.nc[1]; ++cy)
+        for (int cz = 0; cz < s.nc[2]; ++cz)
+          hcc[cellmap[s.cellmap_off + (cx * s.nc[1] + cy) * s.nc[2] + cz]] =
+              make_double4(cx * s.hc[0], cy * s.hc[1], cz * s.hc[2], 0.0);
     // regions (need the complete cell map of the system)
     for (int k = s.brick_off; k < (int)hb.size(); ++k) {
       BBrick& b = hb[k];
@@ -2047,6 +2145,12 @@ static int build_tables(BMD& m) {
   HIP_TRY(hipMemcpy(m.d_reg, hr.data(), hr.size() * sizeof(BRegion), hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&m.d_cellmap, cellmap.size() * sizeof(int)));
   HIP_TRY(hipMemcpy(m.d_cellmap, cellmap.data(), cellmap.size() * sizeof(int), hipMemcpyHostToDevice));
+  {
+    double4* cc = nullptr;
+    HIP_TRY(hipMalloc(&cc, hcc.size() * sizeof(double4)));
+    HIP_TRY(hipMemcpy(cc, hcc.data(), hcc.size() * sizeof(double4), hipMemcpyHostToDevice));
+    m.sta.ccorner = cc;
+  }
   HIP_TRY(hipMalloc(&m.d_cellsys, cellsys.size() * sizeof(int)));
   HIP_TRY(hipMemcpy(m.d_cellsys, cellsys.data(), cellsys.size() * sizeof(int), hipMemcpyHostToDevice));
   std::vector<int> sysid(m.N);
@@ -2088,6 +2192,8 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
   void** i4s[] = {(void**)&m->sta.perm[0], (void**)&m->sta.perm[1], (void**)&m->sta.cell[0], (void**)&m->sta.cell[1],
                   (void**)&m->tmp,         (void**)&m->slot,        (void**)&m->nn};
   for (void** p : i4s) HIP_TRY(hipMalloc(p, i4 + 64));
+  if (precision == ACCU_FP32_PAIR)
+    for (int k = 0; k < 2; ++k) HIP_TRY(hipMalloc(&m->sta.posf[k], (size_t)N * sizeof(int4) + 64));
   HIP_TRY(hipMalloc(&m->par, nsys * sizeof(int) + 64));
   HIP_TRY(hipMemset(m->par, 0, nsys * sizeof(int) + 64));
   HIP_TRY(hipMalloc(&m->spar, nsys * sizeof(int) + 64));

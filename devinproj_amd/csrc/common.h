@@ -65,6 +65,18 @@ __device__ __forceinline__ double minimg_fast(double d, double L, double inv) {
   return fma(-L, rint(d * inv), d);
 }
 
+// numpy's x % L for float64 (the periodic wrap of system.py:127-129): fmod, moved into
+// [0, L) by adding L when the signs differ, and +0.0 for an exact multiple.
+__device__ __forceinline__ double np_mod2(double x, double L) {
+  double r = fmod(x, L);
+  if (r != 0.0) {
+    if ((r < 0.0) != (L < 0.0)) r += L;
+  } else {
+    r = copysign(0.0, L);
+  }
+  return r;
+}
+
 // |d|^2 summed as numpy's einsum("ij,ij->i") sums a 3-vector: (x^2 + z^2) + y^2.
 __device__ __forceinline__ double r2_np(double x, double y, double z) {
 #pragma clang fp contract(off)

@@ -117,6 +117,34 @@ int accu_bh_info(accu_bh* h, long long* out_host);
 int accu_bh_export(accu_bh* h, double* cube, double* com, double* quad, int* next, int* leaf, int* start,
                    int* count, int* perm, void* stream);
 
+/* ---- device-resident N-body runs: Simulation.run (simulation.py:95-112) with
+ * VelocityVerlet / Yoshida4 (integrators.py:19-47) over direct Gravity (forces.py:31-48)
+ * or BarnesHut (tree.py:19-78). Host arrays in user order; box_host may be NULL (open
+ * space) or a periodic box. Direct gravity fuses the force sum, the closing kick, the
+ * record sums and the next opening kick and drift into one launch per substep. */
+typedef struct accu_nbody accu_nbody;
+int accu_nbody_create(accu_nbody** out, int n, const double* pos_host, const double* vel_host,
+                      const double* mass_host, const double* box_host, int periodic);
+int accu_nbody_destroy(accu_nbody* h);
+/* Gravity (forces.py:31-48): precision ACCU_FP64, or ACCU_FP32_PAIR (open space; pair
+ * terms relative to the initial centre of mass, sums in float64) */
+int accu_nbody_set_direct(accu_nbody* h, double G, double eps2, int precision);
+/* BarnesHut (tree.py:19-78), open space; quadrupole != 0 adds quadrupole terms */
+int accu_nbody_set_bh(accu_nbody* h, double G, double eps2, double theta, int leaf_size, int quadrupole);
+/* forces at t = 0 and record 0; log_cap records are kept */
+int accu_nbody_init(accu_nbody* h, int log_cap);
+/* steps steps of dt, each the substeps of weights w_host[0..nsub) (1 for VelocityVerlet;
+ * w1, w0, w1 for Yoshida4); a record every record_every steps and at the last;
+ * use_graph replays one captured hipGraph (direct gravity only) */
+int accu_nbody_run(accu_nbody* h, int steps, double dt, const double* w_host, int nsub, int record_every,
+                   int use_graph);
+int accu_nbody_sync(accu_nbody* h);
+/* state (host [n][3], any may be NULL): positions, velocities, accelerations F / m;
+ * log_host [records][8] = PE, KE, Px, Py, Pz, Lx, Ly, Lz; synchronises */
+int accu_nbody_get(accu_nbody* h, double* pos_host, double* vel_host, double* acc_host, double* log_host,
+                   int log_max, long long* nrec_out);
+int accu_nbody_stream(accu_nbody* h, void** stream_out);
+
 /* ---- device-resident MD: Simulation.run (simulation.py:95-112) driving
  * VelocityVerlet / Yoshida4 (integrators.py:19-47) over LJ (forces.py:119-142) and/or
  * DSF (forces.py:408-438), with the Verlet rebuild rule of forces.py:86-98 applied
