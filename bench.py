@@ -151,33 +151,38 @@ def drift_long(s, precision, steps=10_000, transient=1_000):
     return out, E, scale
 
 
-def richardson_batch(coarse_steps=2000, dt=0.005, precision="mixed"):
+def richardson_batch(coarse_steps=2000, short_steps=100, precision="mixed"):
     """48 uneven reduced-unit LJ systems (N log-uniform in [4000, 262144], rho* in
     [0.7, 0.9], T* = 1), each at dt, dt/2, dt/4 (144 trajectories) in one batched engine,
-    `coarse_steps` steps at the coarsest dt, blocks of 10 captured as one hipGraph;
-    timestep_convergence (accuracy.py:191-221) per system."""
-    import devinproj_amd as dp
+    blocks of 10 steps captured as one hipGraph, brick shapes and workgroup size from the
+    autotuner's cache (one tile per size bucket); timestep_convergence (accuracy.py:191-221)
+    per system, over `coarse_steps` coarse steps of dt = 0.005 tau (10 tau: past the
+    Lyapunov time of the liquid) and over `short_steps` (0.5 tau: within it)."""
+    import re
+
     from devinproj_amd import autotune, batch
-    from devinproj_amd import workloads as W
-    systems = [W.lj_reduced(n, rho, T=1.0, seed=sd) for n, rho, sd in W.uneven_batch()]
-    make = lambda sy: [dp.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3, precision=precision)]
-    tiles = [autotune.cached_tile(sy, make(sy), precision, autotune.DEFAULT_CACHE) for sy in systems]
-    bricks = [t["bricks"] if t else None for t in tiles]  # None: the engine's default shape
-    t0 = time.perf_counter()
-    checks, eng, _ = batch.richardson(systems, make, dt, coarse_steps, precision=precision, bricks=bricks,
-                                      use_graph=True, block=10, threads=256)
-    t1 = time.perf_counter()
+    systems, make, dt = autotune.richardson_workload()
+    bricks, threads = autotune.batch_tiles(systems, make, precision, autotune.DEFAULT_CACHE)
     n_atoms = sum(sy.n for sy in systems)
-    atom_steps = n_atoms * coarse_steps * (1 + 2 + 4)
-    status = {}
-    for c in checks:
-        status[c.status] = status.get(c.status, 0) + 1
-    est = sorted(c.value for c in checks if c.value is not None)
     out = {"systems": len(systems), "trajectories": 3 * len(systems), "atoms_per_replica_set": n_atoms,
-           "coarse_steps": coarse_steps, "run_seconds": eng.run_seconds, "total_seconds": t1 - t0,
-           "coarse_steps_per_s": coarse_steps / eng.run_seconds, "atom_steps_per_s": atom_steps / eng.run_seconds,
-           "status_counts": status, "median_estimate": est[len(est) // 2] if est else None}
-    eng.close()
+           "dt": dt, "tuned_systems": sum(b is not None for b in bricks), "threads": threads}
+    for tag, steps in (("long", coarse_steps), ("short", short_steps)):
+        t0 = time.perf_counter()
+        checks, eng, _ = batch.richardson(systems, make, dt, steps, precision=precision, bricks=bricks,
+                                          use_graph=True, block=10, threads=threads)
+        t1 = time.perf_counter()
+        status = {}
+        for c in checks:
+            status[c.status] = status.get(c.status, 0) + 1
+        est = sorted(c.value for c in checks if c.value is not None)
+        orders = sorted(float(m.group(1)) for c in checks for m in [re.search(r"observed order (-?[\d.]+)", c.detail)]
+                        if m)
+        out[tag] = {"coarse_steps": steps, "run_seconds": eng.run_seconds, "total_seconds": t1 - t0,
+                    "coarse_steps_per_s": steps / eng.run_seconds,
+                    "atom_steps_per_s": n_atoms * steps * (1 + 2 + 4) / eng.run_seconds, "status_counts": status,
+                    "median_estimate": est[len(est) // 2] if est else None,
+                    "median_observed_order": orders[len(orders) // 2] if orders else None}
+        eng.close()
     return out
 
 
@@ -257,19 +262,26 @@ def gravity_plummer(n=262144, seed=11):
 
 
 def learned_throughput(reps=5):
-    """Evaluations/s of DSF + learned correction vs Ewald (real-space error 1e-6) on the
-    65,536-ion NaCl snapshot, on the GPU; accuracy is measured by the training script."""
+    """The learned correction on the 65,536-ion NaCl workload, on the GPU: evaluations/s of
+    DSF + the shipped network against Ewald (real-space error 1e-6), each DSF and
+    correction call building its neighbour list afresh (a rebuild every evaluation), and
+    their RMS relative force error against Ewald on a snapshot of an independent melt
+    (Langevin 3000 K then 1200 K, seed 77) that the network never saw, beside DSF alone."""
     import torch
 
     import devinproj_amd as dp
-    from devinproj_amd import learned
-    from devinproj_amd import workloads as W
-    s = W.rocksalt(32, 32, 64, seed=3)
-    rc = 12e-10
-    alpha_e = 3.459 / rc
-    ewald = dp.Ewald(alpha=alpha_e, rcut=rc, kcut=2 * alpha_e * np.sqrt(np.log(1e6)))
-    dsf = dp.DampedShiftedCoulomb(rcut=rc, alpha=0.2e10, precision="mixed")
-    corr = learned.PairCorrection(rc, 2.8e-10, 3.8e-10, F0=1e-10)
+    from devinproj_amd import learned, train_nacl
+    rc = train_nacl.RC
+    s, rms = train_nacl.melt((32, 32, 64), seed=77)
+    ewald = train_nacl.ewald_1e6()
+    corr = learned.load_correction()
+
+    def dsf_fresh(sys_):  # a new instance: its list is built inside the call
+        return dp.DampedShiftedCoulomb(rcut=rc, alpha=train_nacl.ALPHA_DSF, precision="mixed").compute(sys_)
+
+    def learned_fresh(sys_):  # DSF + correction over one list, built inside the call
+        return learned.LearnedDSF(rcut=rc, alpha=train_nacl.ALPHA_DSF, precision="mixed",
+                                  correction=corr).compute(sys_)
 
     def timed(fn):
         fn(s)
@@ -281,11 +293,19 @@ def learned_throughput(reps=5):
         return (time.perf_counter() - t0) / reps
 
     t_e = timed(ewald.compute)
-    t_d = timed(dsf.compute)
-    t_c = timed(corr.compute)
-    return {"ewald_ms": 1e3 * t_e, "dsf_ms": 1e3 * t_d, "correction_ms": 1e3 * t_c,
-            "dsf_plus_correction_over_ewald": t_e / (t_d + t_c),
-            "note": "drop-in compute() wall times including host<->device copies"}
+    t_d = timed(dsf_fresh)
+    t_l = timed(learned_fresh)
+    Fe, Fd, Fl = ewald.compute(s)[0], dsf_fresh(s)[0], learned_fresh(s)[0]
+    from devinproj_amd.accuracy import rms_rel_error
+    return {"ewald_ms": 1e3 * t_e, "dsf_ms": 1e3 * t_d, "dsf_plus_correction_ms": 1e3 * t_l,
+            "dsf_plus_correction_over_ewald": t_e / t_l,
+            "heldout_rms_rel_error_dsf_vs_ewald": rms_rel_error(Fd, Fe),
+            "heldout_rms_rel_error_dsf_plus_correction_vs_ewald": rms_rel_error(Fl, Fe),
+            "heldout_melt_rms_displacement_A": rms * 1e10,
+            "training_record": {k: corr.meta.get(k) for k in ("rms_rel_error_dsf_vs_ewald",
+                                                              "rms_rel_error_dsf_plus_ls_pair_bound_vs_ewald",
+                                                              "rms_rel_error_dsf_plus_correction_vs_ewald")},
+            "note": "drop-in compute() wall times including host<->device copies and the list builds"}
 
 
 def rattle_dimers(n_mol=131072, steps=200, block=20):

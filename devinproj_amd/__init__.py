@@ -6,7 +6,6 @@ The HIP library (_accuhip.so) is required; there is no CPU fallback.
 """
 from .accuracy import AccuracyReport, Check, timestep_convergence
 from .batch import BrickEngine
-from .engine import MDEngine
 from .forces import (DampedShiftedCoulomb, Ewald, Force, Gravity, LennardJones, NeighbourList, VerletList,
                      cell_list_pairs, kvector_columns)
 from .integrators import Brownian, Integrator, Langevin, NoseHoover, Rattle, VelocityVerlet, Yoshida4
@@ -20,7 +19,7 @@ from .units import G, k_B, k_e, si
 
 __all__ = [
     "AccuracyReport", "BarnesHut", "BrickEngine", "Brownian", "Check", "DampedShiftedCoulomb", "Ewald", "Force", "G",
-    "Gravity", "Integrator", "Langevin", "LearnedDSF", "LennardJones", "MDEngine", "NBodyEngine", "NeighbourList", "NoseHoover",
+    "Gravity", "Integrator", "Langevin", "LearnedDSF", "LennardJones", "NBodyEngine", "NeighbourList", "NoseHoover",
     "PairCorrection", "Rattle", "Simulation", "Species", "System",
     "Trajectory", "VelocityVerlet", "train_correction",
     "VerletList", "Yoshida4", "cell_list_pairs", "direct_subset", "k_B", "k_e", "kvector_columns", "si",

@@ -158,17 +158,6 @@ def sync():
 
 
 SIGNATURES.update({
-    "accu_md_create": [P, I, P, P, P, I, P, P, P, I],
-    "accu_md_destroy": [P],
-    "accu_md_set_lj": [P, I, P, P, P, D],
-    "accu_md_set_dsf": [P, D, D, D, D],
-    "accu_md_init": [P, D, D, I, D, I],
-    "accu_md_run": [P, I, D, P, I, I, I],
-    "accu_md_sync": [P],
-    "accu_md_get": [P, P, P, P, P, I, P],
-    "accu_md_stream": [P, P],
-    "accu_md_set_timing": [P, I],
-    "accu_md_timing": [P, P, P],
     "accu_mlp_create": [P, P],
     "accu_mlp_destroy": [P],
     "accu_mlp_params": [P, P, P],

@@ -5,7 +5,7 @@ workgroup size of the per-brick kernels.
 candidate (each brick shape whose staged region fits the LDS, at 256 and 512 threads),
 times whole MD steps on the wall clock (fused steps, records and the rebuilds that
 happen in the run), and returns the fastest as {"bricks": (bx, by, bz), "threads": t}.
-Results are cached per size bucket (atoms per cell, log2 N, precision, interactions) in
+Results are cached per size bucket (atoms per cell, N, precision, interactions) in
 memory and optionally in a JSON file, so a batch of many systems tunes one
 representative per bucket (`tune_batch`). The list layout (16-byte groups of eight
 16-bit entries, one group row per `stride` atoms) is fixed, so the list stride is not
@@ -44,11 +44,16 @@ def _list_radius(system, forces):
 
 
 def bucket_key(system, forces, precision):
+    """Size bucket of a system: precision, interactions, atoms per list cell in bands of 4
+    (what decides which brick regions fit the LDS) and N in factors of 4 (what decides how
+    many bricks fill the 256 CUs)."""
     rc = _list_radius(system, forces)
     nc = np.floor(np.asarray(system.box, float) / rc).astype(int)
     per_cell = system.n / float(np.prod(nc))
     kinds = "+".join(sorted(type(f).__name__ for f in forces))
-    return f"{precision}|{kinds}|cell{int(round(per_cell))}|n2^{int(math.log2(max(system.n, 2)))}"
+    band = 4 * int(round(per_cell / 4))
+    size = 2 * int(math.log2(max(system.n, 2)) // 2)
+    return f"{precision}|{kinds}|cell{band}|n2^{size}"
 
 
 def candidates_for(system, forces, precision, lds_bytes=LDS_BYTES, slack=0.25):
@@ -104,8 +109,9 @@ def cached_tile(system, forces, precision, cache_file):
     return _CACHE.get(bucket_key(system, forces, precision))
 
 
-def choose_tile(system, make_forces, precision, dt, cache_file=None, steps=40, threads=THREADS, log=None):
-    key = bucket_key(system, make_forces(system), precision)
+def choose_tile(system, make_forces, precision, dt, cache_file=None, steps=40, threads=THREADS, log=None,
+                key_suffix=""):
+    key = bucket_key(system, make_forces(system), precision) + key_suffix
     _load(cache_file)
     if key in _CACHE:
         return _CACHE[key]
@@ -134,22 +140,73 @@ def choose_bricks(system, make_forces, precision, dt, cache_file=None, steps=40,
     return choose_tile(system, make_forces, precision, dt, cache_file=cache_file, steps=steps, log=log)["bricks"]
 
 
+def _batch_key(keys, precision):
+    """Cache key of a batch: its multiset of size buckets."""
+    sig = ",".join(f"{k.split('|', 2)[2]}x{keys.count(k)}" for k in sorted(set(keys)))
+    return f"batch|{precision}|{sig}"
+
+
+def time_batch(systems, make_forces, precision, bricks, threads, dt, steps=20):
+    """Seconds per coarse step of the batched timestep_convergence run (batch.richardson: every
+    system at dt, dt/2 and dt/4 in one engine, blocks of 10 steps replayed as a hipGraph)."""
+    from .batch import richardson
+    _, eng, _ = richardson(systems, make_forces, dt, steps, precision=precision, bricks=bricks, use_graph=True,
+                           block=10, threads=threads)
+    try:
+        return eng.run_seconds / steps
+    finally:
+        eng.close()
+
+
 def tune_batch(systems, make_forces, precision, dt, cache_file=None, log=None):
-    """One brick shape per system, tuning one representative (the largest) per bucket;
-    returns (bricks per system, threads), the threads being those of the bucket with the
-    most atoms in total (one workgroup size serves the whole batch)."""
+    """Tiles for a batch of systems that share one launch (and so one workgroup size): for
+    each workgroup size, the fastest brick shape of each size bucket (tuned on the bucket's
+    largest system at that size); then, timed on the batched dt, dt/2, dt/4 run itself, the
+    workgroup size and the choice between those shapes and the engine's own shape rule.
+    Returns (bricks per system, threads) and caches the choice."""
     keys = [bucket_key(s, make_forces(s), precision) for s in systems]
-    rep, atoms = {}, {}
+    bkey = _batch_key(keys, precision)
+    _load(cache_file)
+    rep = {}
     for s, k in zip(systems, keys):
         if k not in rep or s.n > rep[k].n:
             rep[k] = s
-        atoms[k] = atoms.get(k, 0) + s.n
-    for k, s in rep.items():
-        if log:
-            log(f"tuning bucket {k} on a {s.n}-atom system")
-        choose_tile(s, make_forces, precision, dt, cache_file=cache_file, log=log)
-    heavy = max(atoms, key=atoms.get)
-    return [_CACHE[k]["bricks"] for k in keys], _CACHE[heavy]["threads"]
+    if bkey not in _CACHE:
+        best, best_t = None, float("inf")
+        for t in THREADS:
+            for k, s in rep.items():
+                if log:
+                    log(f"tuning bucket {k} at {t} threads on a {s.n}-atom system")
+                choose_tile(s, make_forces, precision, dt, cache_file=cache_file, threads=(t,), log=log,
+                            key_suffix=f"|t{t}")
+            # the per-bucket shapes, and the engine's own shape rule, on the whole batch
+            for tuned in (True, False):
+                bricks = [_CACHE[k + f"|t{t}"]["bricks"] for k in keys] if tuned else None
+                sec = time_batch(systems, make_forces, precision, bricks, t, dt)
+                if log:
+                    log(f"batch of {len(systems)} at {t} threads, {'tuned' if tuned else 'engine default'} "
+                        f"bricks: {1e3 * sec:.3f} ms per step")
+                if sec < best_t:
+                    best, best_t = (t, tuned), sec
+        # "bricks" (1, 1, 1) marks per-bucket shapes, (0, 0, 0) the engine's default rule
+        _CACHE[bkey] = {"bricks": (1, 1, 1) if best[1] else (0, 0, 0), "threads": best[0]}
+        _save(cache_file)
+    return batch_tiles(systems, make_forces, precision, cache_file)
+
+
+def batch_tiles(systems, make_forces, precision, cache_file):
+    """Cached tiles of a batch without tuning: (bricks per system, threads) as tune_batch
+    chose them (None for a system: the engine's default shape), or (None per system, 256)
+    when the batch was never tuned."""
+    _load(cache_file)
+    keys = [bucket_key(s, make_forces(s), precision) for s in systems]
+    bkey = _batch_key(keys, precision)
+    if bkey not in _CACHE:
+        return [None] * len(systems), 256
+    t = _CACHE[bkey]["threads"]
+    if tuple(_CACHE[bkey]["bricks"]) == (0, 0, 0):
+        return [None] * len(systems), t
+    return [_CACHE[k + f"|t{t}"]["bricks"] if k + f"|t{t}" in _CACHE else None for k in keys], t
 
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tiles_mi355x.json")
@@ -172,14 +229,30 @@ def _bench_workloads():
     return [(argon, lj_forces, 10e-15), (nacl, nacl_forces, 1e-15)]
 
 
+def richardson_workload():
+    """The 48 uneven reduced-unit LJ systems of the batched timestep_convergence benchmark,
+    their force factory and coarsest dt."""
+    from . import forces as F
+    from . import workloads as W
+    systems = [W.lj_reduced(n, rho, T=1.0, seed=sd) for n, rho, sd in W.uneven_batch()]
+    return systems, (lambda s: [F.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3, precision="mixed")]), 0.005
+
+
 def main(argv=None):
-    """python -m devinproj_amd.autotune [--cache FILE]: tune the benchmark workloads."""
+    """python -m devinproj_amd.autotune [--cache FILE] [--richardson]: tune the benchmark
+    workloads (and, with --richardson, every size bucket of the 48-system batch)."""
     import argparse
     import sys
     ap = argparse.ArgumentParser()
     ap.add_argument("--cache", default=DEFAULT_CACHE)
+    ap.add_argument("--richardson", action="store_true")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.richardson:
+        systems, make, dt = richardson_workload()
+        bricks, threads = tune_batch(systems, make, "mixed", dt, cache_file=a.cache, log=log)
+        print(json.dumps({"richardson_systems": len(systems), "threads": threads,
+                          "bricks": [list(b) for b in bricks]}), flush=True)
     for system, make, dt in _bench_workloads():
         for prec in ("mixed", "fp64"):
             log(f"{system.n} atoms, {prec}")

@@ -1,10 +1,11 @@
-// Learned DSF -> Ewald pair correction: a 4 -> 64 -> 64 -> 1 MLP per directed pair,
+// Learned DSF -> Ewald pair correction: a 5 -> 64 -> 64 -> 1 MLP per directed pair,
 // layer 2 (and the backward GEMMs) on gfx950 bf16 MFMA with float32 accumulation.
 // This has no counterpart in the reference: it corrects DampedShiftedCoulomb
 // (forces.py:391-452) toward Ewald (forces.py:333-388).
 //
 //   n_i  = sum_j sw(r_ij)                          (cosine switch r_on -> r_off)
-//   x    = [r / rc, q_i q_j, (n_i + n_j) s_n, |n_i - n_j| s_n]   (symmetric in i <-> j)
+//   x    = [r / rc, q_i q_j, (n_i + n_j) s_n, |n_i - n_j| s_n, q_i + q_j]   (symmetric in i <-> j;
+//          q_i + q_j tells like pairs of different species apart, e.g. Na-Na from Cl-Cl)
 //   h1   = silu(W1 x + b1);  h2 = silu(W2 h1 + b2)  (W2, h1 in bfloat16)
 //   s    = (w3 . h2 + b3) (1 - (r/rc)^2)^2
 //   F_i += F0 s d / r,  d = r_j - r_i
@@ -31,8 +32,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 64;
-constexpr int kNParam = kH * 4 + kH + kH * kH + kH + kH + 1;  // 4545
-constexpr int kOffW1 = 0, kOffB1 = 256, kOffW2 = 320, kOffB2 = 4416, kOffW3 = 4480, kOffB3 = 4544;
+constexpr int kIn = 5;  // input features
+constexpr int kNParam = kH * kIn + kH + kH * kH + kH + kH + 1;  // 4609
+constexpr int kOffW1 = 0, kOffB1 = kH * kIn, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH,
+              kOffB3 = kOffW3 + kH;
 constexpr int kQ = 96;  // per-wave queue of compacted pairs
 
 struct MLPGeom {
@@ -81,7 +84,7 @@ __global__ void k_mlp_coord(const double* __restrict__ pos, int ps, int n, const
 
 // Parameters staged in LDS (float32) and the per-lane W2 operands (bf16).
 struct MLPShared {
-  float W1[kH * 4];
+  float W1[kH * kIn];
   float b1[kH];
   float b2[kH];
   float w3[kH];
@@ -89,7 +92,7 @@ struct MLPShared {
 };
 
 __device__ __forceinline__ void stage_params(const float* __restrict__ prm, MLPShared& sh) {
-  for (int k = threadIdx.x; k < kH * 4; k += blockDim.x) sh.W1[k] = prm[kOffW1 + k];
+  for (int k = threadIdx.x; k < kH * kIn; k += blockDim.x) sh.W1[k] = prm[kOffW1 + k];
   for (int k = threadIdx.x; k < kH; k += blockDim.x) {
     sh.b1[k] = prm[kOffB1 + k];
     sh.b2[k] = prm[kOffB2 + k];
@@ -101,7 +104,7 @@ __device__ __forceinline__ void stage_params(const float* __restrict__ prm, MLPS
 // Forward of one tile. Returns s for pair p (both halves) and leaves the layer-2 D
 // values and the features in the caller's registers when asked (training).
 struct TileState {
-  float x[4];
+  float x[kIn];
   float r, env;
   bf16x8 bh1[4];  // h1 of pair p for units kb*16 + hk*8 + t
   f32x16 z2[2];   // layer-2 pre-activations (with bias), D layout
@@ -116,10 +119,8 @@ __device__ __forceinline__ void tile_forward(const MLPShared& sh, const bf16x8 (
     for (int t = 0; t < 8; ++t) {
       const int u = kb * 16 + hk * 8 + t;
       float z = sh.b1[u];
-      z = fmaf(sh.W1[4 * u], st.x[0], z);
-      z = fmaf(sh.W1[4 * u + 1], st.x[1], z);
-      z = fmaf(sh.W1[4 * u + 2], st.x[2], z);
-      z = fmaf(sh.W1[4 * u + 3], st.x[3], z);
+#pragma unroll
+      for (int c = 0; c < kIn; ++c) z = fmaf(sh.W1[kIn * u + c], st.x[c], z);
       st.bh1[kb][t] = (__bf16)silu(z);
     }
   }
@@ -158,6 +159,7 @@ __device__ __forceinline__ void tile_features(const QEntry& e, bool valid, float
   st.x[1] = qi * qj;
   st.x[2] = (ni + nj) * g.ns;
   st.x[3] = fabsf(ni - nj) * g.ns;
+  st.x[4] = qi + qj;
   const float a = 1.f - st.r * st.r * g.inv_rc * g.inv_rc;
   st.env = valid ? a * a : 0.f;
 }
@@ -272,7 +274,7 @@ struct __attribute__((aligned(16))) GradScratch {
   __bf16 dz2[kH * 32];   // [u][p]
   __bf16 dz2T[32 * kH];  // [p][u]
   __bf16 dz1[kH * 32];   // [v][p]
-  __bf16 xT[16 * 32];    // [c][p], c = 4 is the bias column
+  __bf16 xT[16 * 32];    // [c][p], c = kIn is the bias column
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const double* __restrict__ pos
         if (hk == 0) {
 #pragma unroll
           for (int c = 0; c < 16; ++c) {
-            const float xv = !valid ? 0.f : (c < 4 ? st.x[c] : (c == 4 ? 1.f : 0.f));
+            const float xv = !valid ? 0.f : (c < kIn ? st.x[c] : (c == kIn ? 1.f : 0.f));
             S.xT[c * 32 + p] = (__bf16)xv;
           }
         }
@@ -398,10 +400,8 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const double* __restrict__ pos
           for (int r = 0; r < 16; ++r) {
             const int v = mb * 32 + 8 * (r >> 2) + 4 * hk + (r & 3);
             float z1 = sh.b1[v];
-            z1 = fmaf(sh.W1[4 * v], st.x[0], z1);
-            z1 = fmaf(sh.W1[4 * v + 1], st.x[1], z1);
-            z1 = fmaf(sh.W1[4 * v + 2], st.x[2], z1);
-            z1 = fmaf(sh.W1[4 * v + 3], st.x[3], z1);
+#pragma unroll
+            for (int c = 0; c < kIn; ++c) z1 = fmaf(sh.W1[kIn * v + c], st.x[c], z1);
             S.dz1[v * 32 + p] = (__bf16)(dh[r] * dsilu(z1));
           }
         }
@@ -440,8 +440,8 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const double* __restrict__ pos
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int v = vb * 16 + (lane >> 4) * 4 + r, c = lane & 15;
-      if (c < 4) gp[kOffW1 + v * 4 + c] = gW1[vb][r];
-      else if (c == 4) gp[kOffB1 + v] = gW1[vb][r];
+      if (c < kIn) gp[kOffW1 + v * kIn + c] = gW1[vb][r];
+      else if (c == kIn) gp[kOffB1 + v] = gW1[vb][r];
     }
 #pragma unroll
   for (int k = 0; k < 32; ++k) {

@@ -1,5 +1,5 @@
-// Cell / Verlet list state shared by the list builder, the pair-force kernels and the
-// device-resident MD engine.
+// Cell / Verlet list state shared by the list builder and the pair-force, learned-correction
+// and Ewald real-space kernels.
 #pragma once
 #include "common.h"
 
@@ -23,14 +23,14 @@ struct NList {
   long long cap_n = 0;
   int* cell = nullptr;
   int* sorted = nullptr;
-  int* tmp = nullptr;
+  int* tmp = nullptr;   // atom ids 0..n-1 (values of the sort by cell)
+  int* skey = nullptr;  // sorted cell ids
   int* nn = nullptr;
   int* halfcnt = nullptr;
   // per-cell arrays (capacity cap_cells)
   long long cap_cells = 0;
   int* count = nullptr;
   int* start = nullptr;
-  int* cursor = nullptr;
   // ELL neighbour list
   int* nbr = nullptr;
   long long nbr_cap = 0;
@@ -40,18 +40,20 @@ struct NList {
   int* dev_scalars = nullptr;  // [0] = max neighbour count of the last build
   long long* half_off = nullptr;
   long long half_cap = 0;
+  void* scratch = nullptr;  // temporary storage of the library sort and scans
+  size_t scratch_cap = 0;
 
   int ensure(int natoms, long long cells);
+  int ensure_scratch(size_t bytes);
   int ensure_ell(int natoms, int nbrs);
   void release();
 };
 
 int make_geo(CellGeo& g, const double* lo, const double* span, const int* ncell, const double* box, int periodic);
-int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, const int* flag,
-                    hipStream_t st);
+int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, hipStream_t st);
 // exact = 1: minimum image with the true division (bit-exact pair sets, the drop-in
-// path); exact = 0: reciprocal multiply, identical for every |d| < L/2 (engine rebuilds)
+// path); exact = 0: reciprocal multiply, identical for every |d| < L/2
 int launch_build_ell(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, double rc2,
-                     const int* sorted, const int* flag, hipStream_t st, int exact = 1);
+                     const int* sorted, hipStream_t st, int exact = 1);
 
 }  // namespace accu

@@ -15,25 +15,25 @@
 #include "nlist.h"
 #include "../../include/accusim_hip.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 #include <algorithm>
 #include <cstring>
 
 namespace accu {
 
 // ------------------------------------------------------------------ kernels
-// Every rebuild kernel takes `flag`: when non-null and *flag == 0 the kernel exits,
-// so the whole chain can sit in a captured graph and run only when a rebuild is due.
 
-__global__ void k_zero_int(int* __restrict__ a, long long n, const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
+__global__ void k_zero_int(int* __restrict__ a, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
 }
 
+// Flat cell of every atom (forces.py:292-296), its count, and the identity values of the
+// sort that follows.
 __global__ void k_cell_assign(const double* __restrict__ pos, int stride_pos, int n, CellGeo g,
-                              int* __restrict__ cell, int* __restrict__ count,
-                              const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
+                              int* __restrict__ cell, int* __restrict__ count, int* __restrict__ idx) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int c[3];
@@ -53,88 +53,8 @@ __global__ void k_cell_assign(const double* __restrict__ pos, int stride_pos, in
   }
   int flat = (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2];
   cell[i] = flat;
+  idx[i] = i;
   atomicAdd(&count[flat], 1);
-}
-
-// Exclusive scan of n ints with one 1024-thread block; out[n] = total.
-__global__ void __launch_bounds__(1024) k_scan_excl(const int* __restrict__ in, int* __restrict__ out,
-                                                    int* __restrict__ out2, long long n,
-                                                    const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
-  __shared__ long long s[1024];
-  const int t = threadIdx.x;
-  const long long per = (n + 1023) / 1024;
-  const long long a = t * per, b = (a + per < n) ? a + per : n;
-  long long sum = 0;
-  for (long long i = a; i < b; ++i) sum += in[i];
-  s[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    long long v = (t >= o) ? s[t - o] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
-  }
-  long long run = s[t] - sum;  // exclusive prefix of this chunk
-  for (long long i = a; i < b; ++i) {
-    int v = in[i];
-    out[i] = (int)run;
-    if (out2) out2[i] = (int)run;
-    run += v;
-  }
-  if (t == 1023) {
-    out[n] = (int)s[1023];
-    if (out2) out2[n] = (int)s[1023];
-  }
-}
-
-__global__ void k_scan_excl64(const int* __restrict__ in, long long* __restrict__ out, long long n) {
-  __shared__ long long s[1024];
-  const int t = threadIdx.x;
-  const long long per = (n + 1023) / 1024;
-  const long long a = t * per, b = (a + per < n) ? a + per : n;
-  long long sum = 0;
-  for (long long i = a; i < b; ++i) sum += in[i];
-  s[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    long long v = (t >= o) ? s[t - o] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
-  }
-  long long run = s[t] - sum;
-  for (long long i = a; i < b; ++i) {
-    out[i] = run;
-    run += in[i];
-  }
-  if (t == 1023) out[n] = s[1023];
-}
-
-__global__ void k_cell_scatter(const int* __restrict__ cell, int n, int* __restrict__ cursor,
-                               int* __restrict__ tmp, const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int slot = atomicAdd(&cursor[cell[i]], 1);
-  tmp[slot] = i;
-}
-
-// Within each cell, order atoms by id (= numpy's stable argsort of the flat cell ids).
-// One wave per cell; rank of each element = number of smaller ids in the segment.
-__global__ void k_cell_sort(const int* __restrict__ start, long long ncells, const int* __restrict__ tmp,
-                            int* __restrict__ sorted, const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
-  long long c = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (c >= ncells) return;
-  const int lane = threadIdx.x & 63;
-  const int a = start[c], b = start[c + 1], len = b - a;
-  for (int e = lane; e < len; e += 64) {
-    int v = tmp[a + e];
-    int r = 0;
-    for (int x = 0; x < len; ++x) r += (tmp[a + x] < v);
-    sorted[a + r] = v;
-  }
 }
 
 __device__ __forceinline__ void unflat(int f, const int* nc, int* c) {
@@ -149,9 +69,7 @@ __device__ __forceinline__ void unflat(int f, const int* nc, int* c) {
 __global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int n, CellGeo g,
                             double rc2, const int* __restrict__ cell, const int* __restrict__ start,
                             const int* __restrict__ sorted, int* __restrict__ nn, int* __restrict__ nbr,
-                            int stride, int max_nbrs, int* __restrict__ maxcount,
-                            const int* __restrict__ flag, int exact) {
-  if (flag && *flag == 0) return;
+                            int stride, int max_nbrs, int* __restrict__ maxcount, int exact) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   double inv[3] = {1.0 / g.box[0], 1.0 / g.box[1], 1.0 / g.box[2]};
   if (t >= n) return;
@@ -220,6 +138,7 @@ __global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int 
 __global__ void k_count_half(int n, const int* __restrict__ sorted, const int* __restrict__ nn,
                              const int* __restrict__ nbr, int stride, int* __restrict__ cnt) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) cnt[n] = 0;  // the scan's extra slot gives the total
   if (t >= n) return;
   const int i = sorted ? sorted[t] : t;
   int c = 0;
@@ -280,14 +199,14 @@ int NList::ensure(int natoms, long long cells) {
   int rc;
   if (natoms + 1 > cap_n) {
     long long c = natoms + 1 + natoms / 4 + 64;
-    int** arrs[] = {&cell, &sorted, &tmp, &nn, &halfcnt};
+    int** arrs[] = {&cell, &sorted, &tmp, &skey, &nn, &halfcnt};
     for (int** a : arrs)
       if ((rc = realloc_ints(a, c))) return rc;
     cap_n = c;
   }
   if (cells + 1 > cap_cells) {
     long long c = cells + 1 + cells / 4 + 64;
-    int** arrs[] = {&count, &start, &cursor};
+    int** arrs[] = {&count, &start};
     for (int** a : arrs)
       if ((rc = realloc_ints(a, c))) return rc;
     cap_cells = c;
@@ -313,33 +232,54 @@ int NList::ensure_ell(int natoms, int nbrs) {
 }
 
 void NList::release() {
-  int* ps[] = {cell, sorted, tmp, nn, halfcnt, count, start, cursor, nbr, dev_scalars};
+  int* ps[] = {cell, sorted, tmp, skey, nn, halfcnt, count, start, nbr, dev_scalars};
   for (int* p : ps)
     if (p) (void)hipFree(p);
   if (half_off) (void)hipFree(half_off);
+  if (scratch) (void)hipFree(scratch);
 }
 
-// Run the binning + sort part of a rebuild (all kernels flag-gated).
-int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, const int* flag,
-                    hipStream_t st) {
+int NList::ensure_scratch(size_t bytes) {
+  if (bytes <= scratch_cap && scratch) return ACCU_OK;
+  if (scratch) HIP_TRY(hipFree(scratch));
+  scratch = nullptr;
+  HIP_TRY(hipMalloc(&scratch, bytes + 4096));
+  scratch_cap = bytes + 4096;
+  return ACCU_OK;
+}
+
+// Binning and the stable sort by cell (forces.py:292-298: np.argsort(cell, kind="stable")):
+// counts per cell (atomics: the count is order-free), their exclusive scan to cell starts
+// (start[ncells] = n), and a stable radix sort of (cell, atom) pairs, so atoms of a cell
+// keep their index order.
+int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, hipStream_t st) {
   const int T = 256;
   const long long nc = h.ncells;
-  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nc + 1 + T - 1) / T)), dim3(T), 0, st, h.count, nc + 1, flag);
+  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nc + 1 + T - 1) / T)), dim3(T), 0, st, h.count, nc + 1);
   hipLaunchKernelGGL(k_cell_assign, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, h.cell, h.count,
-                     flag);
-  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, h.count, h.start, h.cursor, nc, flag);
-  hipLaunchKernelGGL(k_cell_scatter, dim3((n + T - 1) / T), dim3(T), 0, st, h.cell, n, h.cursor, h.tmp, flag);
-  hipLaunchKernelGGL(k_cell_sort, dim3((unsigned)((nc + 3) / 4)), dim3(256), 0, st, h.start, nc, h.tmp, h.sorted,
-                     flag);
+                     h.tmp);
   LAUNCH_CHECK();
+  size_t b1 = 0, b2 = 0;
+  HIP_TRY(rocprim::exclusive_scan(nullptr, b1, h.count, h.start, 0, (size_t)nc + 1, rocprim::plus<int>(), st));
+  unsigned bits = 1;
+  while (bits < 31 && (1ll << bits) < nc) ++bits;
+  // cell ids are non-negative: sorted as unsigned keys over their low `bits` bits
+  const unsigned* keys_in = reinterpret_cast<const unsigned*>(h.cell);
+  unsigned* keys_out = reinterpret_cast<unsigned*>(h.skey);
+  HIP_TRY(rocprim::radix_sort_pairs(nullptr, b2, keys_in, keys_out, h.tmp, h.sorted, (unsigned)n, 0u, bits, st));
+  int rc = h.ensure_scratch(b1 > b2 ? b1 : b2);
+  if (rc) return rc;
+  b1 = b2 = h.scratch_cap;
+  HIP_TRY(rocprim::exclusive_scan(h.scratch, b1, h.count, h.start, 0, (size_t)nc + 1, rocprim::plus<int>(), st));
+  HIP_TRY(rocprim::radix_sort_pairs(h.scratch, b2, keys_in, keys_out, h.tmp, h.sorted, (unsigned)n, 0u, bits, st));
   return ACCU_OK;
 }
 
 int launch_build_ell(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, double rc2,
-                     const int* sorted, const int* flag, hipStream_t st, int exact) {
+                     const int* sorted, hipStream_t st, int exact) {
   const int T = 256;
   hipLaunchKernelGGL(k_build_ell, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, rc2, h.cell, h.start,
-                     sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, flag, exact);
+                     sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, exact);
   LAUNCH_CHECK();
   return ACCU_OK;
 }
@@ -405,12 +345,12 @@ int accu_nlist_build(accu_nlist* hh, const double* pos, int n, const double* lo,
     h.last_max = 0;
     return ACCU_OK;
   }
-  if ((rc_ = launch_bin_sort(h, pos, 3, n, g, nullptr, st))) return rc_;
+  if ((rc_ = launch_bin_sort(h, pos, 3, n, g, st))) return rc_;
   int guess = h.max_nbrs > 0 ? h.max_nbrs : 64;
   for (int attempt = 0; attempt < 3; ++attempt) {
     if ((rc_ = h.ensure_ell(n, guess))) return rc_;
     HIP_TRY(hipMemsetAsync(h.dev_scalars, 0, sizeof(int), st));
-    if ((rc_ = launch_build_ell(h, pos, 3, n, g, h.rc2, h.sorted, nullptr, st))) return rc_;
+    if ((rc_ = launch_build_ell(h, pos, 3, n, g, h.rc2, h.sorted, st))) return rc_;
     int mx = 0;
     HIP_TRY(hipMemcpyAsync(&mx, h.dev_scalars, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -461,8 +401,15 @@ int accu_nlist_count_half(accu_nlist* hh, long long* npairs, void* stream) {
   const int T = 256;
   hipLaunchKernelGGL(k_count_half, dim3((h.n + T - 1) / T), dim3(T), 0, st, h.n, h.sorted, h.nn, h.nbr, h.stride,
                      h.halfcnt);
-  hipLaunchKernelGGL(k_scan_excl64, dim3(1), dim3(1024), 0, st, h.halfcnt, h.half_off, (long long)h.n);
   LAUNCH_CHECK();
+  size_t bytes = 0;
+  HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, h.halfcnt, h.half_off, 0ll, (size_t)h.n + 1,
+                                  rocprim::plus<long long>(), st));
+  int rc = h.ensure_scratch(bytes);
+  if (rc) return rc;
+  bytes = h.scratch_cap;
+  HIP_TRY(rocprim::exclusive_scan(h.scratch, bytes, h.halfcnt, h.half_off, 0ll, (size_t)h.n + 1,
+                                  rocprim::plus<long long>(), st));
   HIP_TRY(hipMemcpyAsync(npairs, h.half_off + h.n, sizeof(long long), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return ACCU_OK;

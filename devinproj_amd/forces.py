@@ -287,25 +287,31 @@ class DampedShiftedCoulomb(_DeviceForce):
     def rebuilds(self):
         return self._vl.rebuilds
 
-    def compute(self, s):
-        q = s.charges
-        if not np.any(q):
-            return np.zeros_like(s.pos), 0.0
+    def device_forces(self, s):
+        """The forces on the device: (F, U, positions) device buffers and the neighbour
+        list they used (None for the all-pairs path); nothing is copied back."""
         th = L.torch()
         a, rc = self.alpha, self.rcut
         erc = float(_erfc(np.array(a * rc)))
         pos_d = self._pos(s)
-        q_d = self._buf.upload("q", np.asarray(q, float), th.float64)
+        q_d = self._buf.upload("q", np.asarray(s.charges, float), th.float64)
         keys = _exclusion_keys(self.exclude, s)
         ex_d = self._buf.upload("ex", keys, th.int64) if len(keys) else None
-        h = None
+        nl = None
         if s.n >= self.pairlist_threshold:
             self._vl.update(s, pos_d, rc, self.skin)
-            h = self._vl.nl.handle
+            nl = self._vl.nl
         box, periodic = _box_args(s)
         F, U = self._out(s.n)
-        L.call("accu_coulomb_forces", h, L.ptr(pos_d), L.ptr(q_d), s.n, 1, float(a), float(rc), erc, float(self.k),
-               L.ptr(ex_d), len(keys), L.ptr(box), periodic, self._prec, L.ptr(F), L.ptr(U), L.stream())
+        L.call("accu_coulomb_forces", None if nl is None else nl.handle, L.ptr(pos_d), L.ptr(q_d), s.n, 1, float(a),
+               float(rc), erc, float(self.k), L.ptr(ex_d), len(keys), L.ptr(box), periodic, self._prec, L.ptr(F),
+               L.ptr(U), L.stream())
+        return F, U, pos_d, nl
+
+    def compute(self, s):
+        if not np.any(s.charges):
+            return np.zeros_like(s.pos), 0.0
+        F, U, _, _ = self.device_forces(s)
         return self._fetch(F, U)
 
 

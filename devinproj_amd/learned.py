@@ -1,9 +1,10 @@
 """Learned DSF -> Ewald pair correction (no counterpart in the reference).
 
-`PairCorrection` is a small antisymmetric pair MLP, [r, q_i q_j, n_i, n_j] -> 64 -> 64
--> 1, whose scalar output acts along r_ij. The coordination numbers n_i enter
-symmetrised as (n_i + n_j, |n_i - n_j|), so s_ij == s_ji and the correction forces
-cancel in pairs (momentum is conserved). Because n_i depends on positions and is not
+`PairCorrection` is a small antisymmetric pair MLP, [r, q_i q_j, n_i, n_j, q_i + q_j] ->
+64 -> 64 -> 1, whose scalar output acts along r_ij. The coordination numbers n_i enter
+symmetrised as (n_i + n_j, |n_i - n_j|), and q_i + q_j tells like pairs of different
+species apart (Na-Na from Cl-Cl, which share q_i q_j), so s_ij == s_ji and the correction
+forces cancel in pairs (momentum is conserved). Because n_i depends on positions and is not
 differentiated, the correction is not the gradient of an energy; it reports U = 0 and
 `conservative = False`.
 
@@ -24,18 +25,24 @@ from .forces import DampedShiftedCoulomb, Ewald, Force, NeighbourList, _box_args
 from .units import e_charge
 
 H = 64
-N_PARAM = H * 4 + H + H * H + H + H + 1
-OFFSETS = {"W1": (0, (H, 4)), "b1": (256, (H,)), "W2": (320, (H, H)), "b2": (4416, (H,)), "w3": (4480, (H,)),
-           "b3": (4544, (1,))}
+N_IN = 5
+N_PARAM = H * N_IN + H + H * H + H + H + 1
+_W1, _B1 = 0, H * N_IN
+_W2 = _B1 + H
+_B2 = _W2 + H * H
+_W3 = _B2 + H
+_B3 = _W3 + H
+OFFSETS = {"W1": (_W1, (H, N_IN)), "b1": (_B1, (H,)), "W2": (_W2, (H, H)), "b2": (_B2, (H,)), "w3": (_W3, (H,)),
+           "b3": (_B3, (1,))}
 
 
 def init_params(seed=0):
-    """Flat float32 parameter vector: W1 ~ N(0, 1/4), W2 ~ N(0, 1/64), w3 ~ N(0, (0.1/8)^2), zero biases."""
+    """Flat float32 parameter vector: W1 ~ N(0, 1/5), W2 ~ N(0, 1/64), w3 ~ N(0, (0.1/8)^2), zero biases."""
     rng = np.random.default_rng(seed)
     p = np.zeros(N_PARAM, np.float32)
-    p[0:256] = rng.normal(0, 1 / np.sqrt(4), 256)
-    p[320:4416] = rng.normal(0, 1 / np.sqrt(H), H * H)
-    p[4480:4544] = rng.normal(0, 0.1 / np.sqrt(H), H)
+    p[_W1:_B1] = rng.normal(0, 1 / np.sqrt(N_IN), H * N_IN)
+    p[_W2:_B2] = rng.normal(0, 1 / np.sqrt(H), H * H)
+    p[_W3:_B3] = rng.normal(0, 0.1 / np.sqrt(H), H)
     return p
 
 
@@ -85,21 +92,27 @@ class PairCorrection(Force):
         p = np.ascontiguousarray(p, np.float32)
         L.call("accu_mlp_params", self._h, None, L.ptr(p))
 
-    def prepare(self, s, pos_d=None):
-        """Upload positions/charges, build the list at max(rcut, r_off) and the coordination numbers."""
+    def prepare(self, s, pos_d=None, nl=None):
+        """Upload positions/charges and compute the coordination numbers over a neighbour
+        list: `nl` when given (any list of these positions with radius >= max(rcut, r_off),
+        e.g. the DSF force's own), else a list built here at that radius."""
         th = L.torch()
         if pos_d is None:
             pos_d = self._buf.upload("pos", np.asarray(s.pos, float), th.float64)
         q_d = self._buf.upload("q", np.asarray(s.charges, float), th.float64)
-        self.nl.build(s, pos_d, max(self.rcut, self.r_off) * (1 + 1e-9))
+        if nl is None:
+            self.nl.build(s, pos_d, max(self.rcut, self.r_off) * (1 + 1e-9))
+            nl = self.nl
+        self._cur = nl
         box, periodic = _box_args(s)
-        L.call("accu_mlp_prepare", self._h, self.nl.handle, L.ptr(pos_d), L.ptr(q_d), 1.0 / self.charge_unit, s.n,
+        L.call("accu_mlp_prepare", self._h, nl.handle, L.ptr(pos_d), L.ptr(q_d), 1.0 / self.charge_unit, s.n,
                L.ptr(box), periodic, L.stream())
         return pos_d
 
     def add_forces(self, s, pos_d, F_d, accumulate=True):
+        """Correction forces over the list of the last prepare(), into F_d."""
         box, periodic = _box_args(s)
-        L.call("accu_mlp_forces", self._h, self.nl.handle, L.ptr(pos_d), s.n, L.ptr(box), periodic,
+        L.call("accu_mlp_forces", self._h, self._cur.handle, L.ptr(pos_d), s.n, L.ptr(box), periodic,
                int(bool(accumulate)), L.ptr(F_d), L.stream())
 
     def compute(self, s):
@@ -117,7 +130,7 @@ class PairCorrection(Force):
         g = np.empty(N_PARAM, np.float32) if want_grad else None
         loss = L.C.c_double(0.0)
         box, periodic = _box_args(s)
-        L.call("accu_mlp_grad", self._h, self.nl.handle, L.ptr(pos_d), s.n, L.ptr(box), periodic, L.ptr(t_d),
+        L.call("accu_mlp_grad", self._h, self._cur.handle, L.ptr(pos_d), s.n, L.ptr(box), periodic, L.ptr(t_d),
                L.ptr(g), L.C.byref(loss), L.stream())
         return loss.value, g
 
@@ -135,7 +148,10 @@ class PairCorrection(Force):
 
 
 class LearnedDSF(DampedShiftedCoulomb):
-    """DampedShiftedCoulomb plus a trained PairCorrection (forces only; U is the DSF energy)."""
+    """DampedShiftedCoulomb plus a trained PairCorrection (forces only; U is the DSF energy).
+    The correction is added on the device into the DSF forces, over the DSF's own Verlet
+    list when it has one (its radius rc + skin covers the correction's rcut and r_off), so
+    one list serves both and one array is copied back."""
 
     conservative = False
 
@@ -145,11 +161,14 @@ class LearnedDSF(DampedShiftedCoulomb):
         self.correction = correction
 
     def compute(self, s):
-        F, U = super().compute(s)
-        if self.correction is not None:
-            Fc, _ = self.correction.compute(s)
-            F = F + Fc
-        return F, U
+        c = self.correction
+        if c is None or not np.any(s.charges):
+            return super().compute(s)
+        F, U, pos_d, nl = self.device_forces(s)
+        radius = self.rcut + self.skin
+        c.prepare(s, pos_d, nl if nl is not None and radius >= max(c.rcut, c.r_off) else None)
+        c.add_forces(s, pos_d, F, accumulate=True)
+        return self._fetch(F, U)
 
 
 NACL_CORRECTION = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "nacl_correction.npz")

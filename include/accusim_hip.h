@@ -145,39 +145,11 @@ int accu_nbody_get(accu_nbody* h, double* pos_host, double* vel_host, double* ac
                    int log_max, long long* nrec_out);
 int accu_nbody_stream(accu_nbody* h, void** stream_out);
 
-/* ---- device-resident MD: Simulation.run (simulation.py:95-112) driving
- * VelocityVerlet / Yoshida4 (integrators.py:19-47) over LJ (forces.py:119-142) and/or
- * DSF (forces.py:408-438), with the Verlet rebuild rule of forces.py:86-98 applied
- * on the device each step and the record() sums of simulation.py:76-93 (PE, KE,
- * momentum) reduced in float64 on the device.
- * create: host arrays in user order, species tables (mass, charge) host [ns]. */
-typedef struct accu_md accu_md;
-int accu_md_create(accu_md** out, int n, const double* pos_host, const double* vel_host, const int* sp_host, int ns,
-                   const double* mass_host, const double* charge_host, const double* box_host, int periodic);
-int accu_md_destroy(accu_md* h);
-int accu_md_set_lj(accu_md* h, int ns, const double* eps_host, const double* sig2_host, const double* rc2_host,
-                   double shift);
-int accu_md_set_dsf(accu_md* h, double alpha, double rcut, double erc, double k);
-/* rc_list includes the skin; capacity = max neighbours at the first build * (1 + slack) + 16 */
-int accu_md_init(accu_md* h, double rc_list, double skin, int precision, double slack, int log_cap);
-/* steps steps of dt split into substeps w_host[0..nsub) * dt; a record every
- * record_every steps and at the last; use_graph replays one captured hipGraph */
-int accu_md_run(accu_md* h, int steps, double dt, const double* w_host, int nsub, int record_every, int use_graph);
-int accu_md_sync(accu_md* h);
-/* state in user order (host [n][3], any may be NULL); log_host: [records][8] =
- * PE, KE, Px, Py, Pz, rebuilds so far, 0, 0; stats: rebuilds, records, max
- * neighbours seen, capacity, list entries. Returns ACCU_EOVERFLOW if the list capacity was exceeded. */
-int accu_md_get(accu_md* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
-                long long* stats);
-int accu_md_stream(accu_md* h, void** stream_out);
-/* HIP-event timing of every fused launch of host-driven runs: total ms and launches */
-int accu_md_set_timing(accu_md* h, int enable);
-int accu_md_timing(accu_md* h, double* total_ms_host, long long* launches_host);
-
-/* ---- brick-tiled multi-system MD engine: the same step as accu_md_* (Simulation.run,
- * simulation.py:95-112, with VelocityVerlet / Yoshida4, integrators.py:19-47, over LJ
- * forces.py:119-142 and/or DSF forces.py:408-438, rebuild rule forces.py:86-98) for
- * one or many periodic systems at once, each with its own box, interactions and dt -
+/* ---- brick-tiled device-resident MD engine: Simulation.run (simulation.py:95-112) with
+ * VelocityVerlet / Yoshida4 (integrators.py:19-47) over LJ (forces.py:119-142) and/or DSF
+ * (forces.py:408-438), the Verlet rebuild rule of forces.py:86-98 applied on the device
+ * every step and the record() sums of simulation.py:76-93 reduced in float64 on the
+ * device, for one or many periodic systems at once, each with its own box, interactions and dt -
  * e.g. the dt, dt/2, dt/4 re-runs of timestep_convergence (accuracy.py:191-221).
  * Systems are given in one concatenated arena (user order); each needs
  * accu_bmd_set_system and the interaction calls before accu_bmd_init. */
@@ -277,8 +249,8 @@ int accu_vacf(const double* frames, int nframes, long long stride, int n, int ma
 
 /* ---- learned DSF -> Ewald pair correction (no counterpart in the reference; it
  * corrects DampedShiftedCoulomb, forces.py:391-452, toward Ewald, forces.py:333-388).
- * A 4 -> 64 -> 64 -> 1 pair MLP; parameters are 4545 float32 values laid out as
- * W1[64][4], b1[64], W2[64][64], b2[64], w3[64], b3. Layer 2 and the backward GEMMs
+ * A 5 -> 64 -> 64 -> 1 pair MLP; parameters are 4609 float32 values laid out as
+ * W1[64][5], b1[64], W2[64][64], b2[64], w3[64], b3. Layer 2 and the backward GEMMs
  * run on bf16 MFMA with float32 accumulation. */
 typedef struct accu_mlp accu_mlp;
 int accu_mlp_create(accu_mlp** out, const float* params_host);
@@ -295,7 +267,7 @@ int accu_mlp_prepare(accu_mlp* h, accu_nlist* h_nl, const double* pos, const dou
 int accu_mlp_forces(accu_mlp* h, accu_nlist* h_nl, const double* pos, int n, const double* box_host, int periodic,
                     int accumulate, double* F, void* stream);
 /* gradient of the loss mean(((F_corr - target) / F0)^2) for one snapshot; target device
- * float64 [n][3]; grad_host [4545] and loss_host may be NULL (else this synchronises) */
+ * float64 [n][3]; grad_host [4609] and loss_host may be NULL (else this synchronises) */
 int accu_mlp_grad(accu_mlp* h, accu_nlist* h_nl, const double* pos, int n, const double* box_host, int periodic,
                   const double* target, float* grad_host, double* loss_host, void* stream);
 /* one Adam step with the last gradient */

@@ -7,7 +7,8 @@ forward/backward kernels with these functions.
 
 Model (per directed pair i -> j within rc, d = r_j - r_i, r = |d|):
   n_i   = sum_{j != i} sw(r_ij),  sw = 1 below r_on, cosine to 0 at r_off
-  x     = [r/rc, q_i q_j / e^2, (n_i + n_j)/n_scale, |n_i - n_j|/n_scale]   (symmetric in i, j)
+  x     = [r/rc, q_i q_j / e^2, (n_i + n_j)/n_scale, |n_i - n_j|/n_scale, (q_i + q_j)/e]
+          (symmetric in i, j)
   h1    = silu(W1 x + b1)            (64)
   h2    = silu(W2 h1 + b2)           (64; W2 and h1 rounded to bfloat16 on the GPU)
   s     = (w3 . h2 + b3) * (1 - (r/rc)^2)^2
@@ -19,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 
 H = 64
-N_IN = 4
+N_IN = 5
 
 
 def bf16(x):
@@ -76,7 +77,8 @@ def coordination(pos, box, r_on, r_off):
 
 
 def features(i, j, r, q, n, rc, e2, n_scale):
-    return np.stack([r / rc, q[i] * q[j] / e2, (n[i] + n[j]) / n_scale, np.abs(n[i] - n[j]) / n_scale], 1)
+    return np.stack([r / rc, q[i] * q[j] / e2, (n[i] + n[j]) / n_scale, np.abs(n[i] - n[j]) / n_scale,
+                     (q[i] + q[j]) / np.sqrt(e2)], 1)
 
 
 def forward(p, x, r, rc, emulate_bf16=True):
@@ -110,10 +112,15 @@ def loss_and_grad(p, n_atoms, i, d, r, x, rc, F0, target, emulate_bf16=True):
     draw = ds * env
     g = {"w3": h2.T @ draw, "b3": np.array([draw.sum()])}
     dz2 = np.outer(draw, p["w3"]) * dsilu(z2)
-    g["W2"] = dz2.T @ h1q
     g["b2"] = dz2.sum(0)
-    dh1 = dz2 @ W2
+    # emulate_bf16: the GPU's backward GEMMs take bfloat16 operands (dz2, dz1 and the
+    # features x), rounded at the same points here
+    dz2q = bf16(dz2) if emulate_bf16 else dz2
+    g["W2"] = dz2q.T @ h1q
+    dh1 = dz2q @ W2
     dz1 = dh1 * dsilu(z1)
+    g["b1"] = (bf16(dz1) if emulate_bf16 else dz1).sum(0)
+    if emulate_bf16:
+        dz1, x = bf16(dz1), bf16(x)
     g["W1"] = dz1.T @ x
-    g["b1"] = dz1.sum(0)
     return L, g, F

@@ -71,6 +71,29 @@ def test_verlet_list_rebuild_decisions_match_oracle():
     assert 2 < vg.rebuilds < 40
 
 
+def test_verlet_list_rebuilds_when_ids_change():
+    """forces.py:88-89: the list is reused only while the particle ids are unchanged. The
+    atoms are re-ordered (same positions, permuted ids), then one id is relabelled, as a
+    reaction would: each must rebuild, in step with the oracle, and the forces of the
+    LennardJones drop-in stay equal to the oracle's (a stale list would scramble them)."""
+    s = B.argon(5, seed=3, jitter=0.02)
+    so = s.copy()
+    f, fo = dp.LennardJones(B.argon_params()), A.LennardJones(B.argon_params())
+    rng = np.random.default_rng(4)
+    for step in range(4):
+        if step == 2:
+            perm = rng.permutation(s.n)
+            for t in (s, so):
+                for name in ("pos", "vel", "sp", "ids"):
+                    setattr(t, name, np.ascontiguousarray(getattr(t, name)[perm]))
+        if step == 3:
+            s.ids[7] = so.ids[7] = 10 ** 6
+        Fg, Ug = f.compute(s)
+        Fo, Uo = fo.compute(so)
+        assert f.rebuilds == fo.rebuilds == [1, 1, 2, 3][step], (step, f.rebuilds, fo.rebuilds)
+        _close(Fg, Ug, Fo, Uo)
+
+
 # ---------------------------------------------------------------- Lennard-Jones
 
 def _close(Fg, Ug, Fo, Uo, rtol=1e-10):

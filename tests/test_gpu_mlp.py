@@ -37,14 +37,18 @@ def _corr(flat, F0=1.0):
 def test_forward_matches_oracle_and_is_antisymmetric():
     s = _system(1)
     flat = learned.init_params(3)
-    flat[256:320] = np.random.default_rng(4).normal(0, 0.3, 64)     # non-zero b1
+    o = learned.OFFSETS["b1"][0]
+    flat[o:o + 64] = np.random.default_rng(4).normal(0, 0.3, 64)     # non-zero b1
     corr = _corr(flat, F0=2.0)
     Fg, U = corr.compute(s)
     assert U == 0.0
     p, i, d, r, x = _oracle(s, flat)
     Fo, _, _ = M.correction_forces(p, s.n, i, d, r, x, RC, 2.0, emulate_bf16=True)
     rel = np.sqrt(np.mean(np.sum((Fg - Fo) ** 2, 1)) / np.mean(np.sum(Fo ** 2, 1)))
-    assert rel < 5e-3, rel
+    print(f"forward rms relative error {rel:.1e}", flush=True)
+    # bfloat16 h1 rounds on float32 inputs on the GPU and float64 here: a value on a
+    # rounding boundary can land one bf16 step apart
+    assert rel < 2e-3, rel
     assert np.max(np.abs(Fg.sum(0))) < 1e-6 * np.max(np.abs(Fg))
 
 
@@ -56,11 +60,14 @@ def test_gradient_matches_oracle():
     loss, g = corr.grad(s, target)
     p, i, d, r, x = _oracle(s, flat)
     Lo, go, _ = M.loss_and_grad(p, s.n, i, d, r, x, RC, 1.0, target, emulate_bf16=True)
-    assert loss == pytest.approx(Lo, rel=1e-3)
+    assert loss == pytest.approx(Lo, rel=1e-4)
     gg = learned.unflatten(g)
-    for k in ("W1", "b1", "W2", "b2", "w3", "b3"):
-        err = np.linalg.norm(gg[k] - go[k]) / np.linalg.norm(go[k])
-        assert err < 3e-2, (k, err)
+    errs = {k: np.linalg.norm(gg[k] - go[k]) / np.linalg.norm(go[k]) for k in ("W1", "b1", "W2", "b2", "w3", "b3")}
+    print("gradient errors per group:", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    for k, err in errs.items():
+        # the oracle rounds the backward GEMM operands to bfloat16 where the kernels do; what
+        # is left is float32 accumulation and the kernels' exp / reciprocal
+        assert err < 1e-3, (k, err)
 
 
 def test_training_reduces_loss_toward_a_teacher():

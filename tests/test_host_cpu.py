@@ -111,7 +111,7 @@ def test_shipped_correction_file_is_plain_data():
     import devinproj_amd
     path = os.path.join(os.path.dirname(devinproj_amd.__file__), "data", "nacl_correction.npz")
     d = np.load(path, allow_pickle=False)
-    assert d["params"].dtype == np.float32 and d["params"].shape == (4545,)
+    assert d["params"].dtype == np.float32 and d["params"].shape == (devinproj_amd.learned.N_PARAM,) == (4609,)
     assert np.all(np.isfinite(d["params"]))
     assert float(d["rc"]) == 12e-10 and float(d["r_on"]) < float(d["r_off"])
     meta = json.loads(str(d["meta"]))

@@ -210,6 +210,27 @@ def test_ewald_nacl_madelung(n_cells):
     assert np.allclose(F, 0, atol=1e-6)
 
 
+def test_ewald_kcut_orthorhombic_madelung():
+    """A 2 x 2 x 4 box (the 1 x 1 x 2 supercell of the conventional NaCl cell) with the
+    |k| <= kcut ellipsoid: the same Madelung constant and zero lattice forces; kcut equal to
+    the cubic test's 2 pi 12 / 2 takes n_z up to 24 along the long axis."""
+    c = B.nacl_cells(1)
+    s = A.System(3, box=np.array([2.0, 2.0, 4.0]), periodic=True)
+    s.add_species("Na", 1.0, 1.0)
+    s.add_species("Cl", 1.0, -1.0)
+    for shift in (0.0, 2.0):
+        for p, sp in zip(c.pos, c.sp):
+            s.add("Na" if sp == 0 else "Cl", [[p[0], p[1], p[2] + shift]])
+    assert s.n == 16 and abs(float(np.sum(s.charges))) == 0.0
+    F, U = A.Ewald(k=1.0, kcut=2 * np.pi * 12 / 2.0).compute(s)
+    assert -U / (s.n / 2) == pytest.approx(1.747564594633, rel=1e-7)
+    assert np.allclose(F, 0, atol=1e-6)
+    # the pin resolves the k set: the reference's |n| <= 12 sphere reaches only half the
+    # |k| along the long axis and misses the constant by far more than 1e-7
+    _, Un = A.Ewald(k=1.0, kmax=12).compute(s)
+    assert abs(-Un / (s.n / 2) / 1.747564594633 - 1) > 1e-5
+
+
 def test_ewald_chunked_and_kcut_agree_with_dense():
     s = B.rocksalt_unit(4)
     rng = np.random.default_rng(2)
