@@ -252,7 +252,7 @@ def main(argv=None):
         systems, make, dt = richardson_workload()
         bricks, threads = tune_batch(systems, make, "mixed", dt, cache_file=a.cache, log=log)
         print(json.dumps({"richardson_systems": len(systems), "threads": threads,
-                          "bricks": [list(b) for b in bricks]}), flush=True)
+                          "bricks": [None if b is None else list(b) for b in bricks]}), flush=True)
     for system, make, dt in _bench_workloads():
         for prec in ("mixed", "fp64"):
             log(f"{system.n} atoms, {prec}")
