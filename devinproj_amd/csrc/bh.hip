@@ -628,7 +628,8 @@ int accu_bh_create(accu_bh** out) {
   accu_bh* h = new accu_bh();
   if (hipMalloc(&h->counter, 64) != hipSuccess || hipMemset(h->counter, 0, 64) != hipSuccess ||
       hipMalloc(&h->root, sizeof(double4)) != hipSuccess || hipMalloc(&h->info, 8 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc(&h->info_h, 8 * sizeof(int)) != hipSuccess) {
+      hipHostMalloc(&h->info_h, 8 * sizeof(int)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {  // the counter fill completes before any caller stream runs
     accu_bh_destroy(h);
     set_error("accu_bh_create: device allocation failed");
     return ACCU_ERUNTIME;

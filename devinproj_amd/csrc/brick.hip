@@ -2230,6 +2230,8 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
   HIP_TRY(hipMemset(m->sta.F[0], 0, d4));
   HIP_TRY(hipMemcpy(m->sta.perm[0], perm.data(), i4, hipMemcpyHostToDevice));
   HIP_TRY(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+  // the fills above ran on the null stream, which a non-blocking stream does not wait for
+  HIP_TRY(hipDeviceSynchronize());
   // species indices are checked when the species table arrives
   m->spec[0].ns = 0;
   *out = reinterpret_cast<accu_bmd*>(m);
@@ -2329,6 +2331,9 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   HIP_TRY(hipMalloc(&m.bsum, BMD::kMaxGrid * sizeof(int)));
   HIP_TRY(hipMalloc(&m.tphase, 16 * sizeof(long long)));
   HIP_TRY(hipMemset(m.tphase, 0, 16 * sizeof(long long)));
+  // null-stream fills complete before the first launch on the engine's non-blocking stream
+  // (a barrier word still being zeroed would leave the rebuild's grid barrier waiting)
+  HIP_TRY(hipDeviceSynchronize());
   // first binning of every system: measure the region sizes to size the LDS
   m.grid_rb = m.resident_grid();
   ARG_CHECK(m.grid_rb > 0, "the rebuild launch (%d threads per block) does not fit on a CU", kRB);
@@ -2595,6 +2600,7 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
     (void)hipFuncSetAttribute((const void*)k_b_rattle<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)m.lds_cons());
   }
+  HIP_TRY(hipDeviceSynchronize());  // null-stream uploads and fills before the engine stream uses them
   return ACCU_OK;
 }
 
@@ -2783,6 +2789,7 @@ int accu_bmd_set_snapshots(accu_bmd* h, int cap_frames, int every) {
     HIP_TRY(hipMalloc(&m.snap_rec, (size_t)cap_frames * sizeof(int)));
     HIP_TRY(hipMemset(m.snap_count, 0, 64));
     HIP_TRY(hipMemset(m.snap_rec, 0xff, (size_t)cap_frames * sizeof(int)));
+    HIP_TRY(hipDeviceSynchronize());  // null-stream fills before the engine stream's next step
     m.snap_cap = cap_frames;
   }
   // the record phase restarts at the next record; a captured graph holds the old ring

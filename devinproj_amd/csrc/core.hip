@@ -19,6 +19,7 @@ int Workspace::ensure(size_t ndoubles) {
   if (!counter) {
     HIP_TRY(hipMalloc(&counter, 64));
     HIP_TRY(hipMemset(counter, 0, 64));
+    HIP_TRY(hipDeviceSynchronize());  // the null-stream fill completes before any caller stream runs
     HIP_TRY(hipMalloc(&out, 16 * sizeof(double)));
   }
   if (ndoubles > cap) {

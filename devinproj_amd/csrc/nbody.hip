@@ -364,7 +364,8 @@ int accu_nbody_create(accu_nbody** out, int n, const double* pos_host, const dou
        hipMemcpy(h->vel, v.data(), d4, hipMemcpyHostToDevice) != hipSuccess ||
        hipMemset(h->acc, 0, d4) != hipSuccess || hipMemset(h->par, 0, 64) != hipSuccess ||
        hipMemset(h->log_count, 0, 64) != hipSuccess ||
-       hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)) {
+       hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess ||
+       hipDeviceSynchronize() != hipSuccess)) {  // null-stream fills before the engine stream runs
     set_error("accu_nbody_create: upload failed");
     rc = ACCU_ERUNTIME;
   }

@@ -214,6 +214,7 @@ int NList::ensure(int natoms, long long cells) {
   if (!dev_scalars) {
     HIP_TRY(hipMalloc(&dev_scalars, 64));
     HIP_TRY(hipMemset(dev_scalars, 0, 64));
+    HIP_TRY(hipDeviceSynchronize());  // the null-stream fill completes before any caller stream runs
   }
   return ACCU_OK;
 }
