@@ -105,10 +105,31 @@ __device__ __forceinline__ double rsqrt64(double x) {
   return y;
 }
 
+// One DPP move of a double (each 32-bit half moved alike); lanes of rows outside
+// ROW_MASK get 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wave, in a fixed order, returned to every lane. All lanes
+// must be active. Cross-lane moves are DPP (quad permutes, row rotations, row
+// broadcasts), so the sum takes no LDS instructions: quads, then 16-lane rows, then
+// row 0 + row 1 and row 2 + row 3, then all four, which lane 63 holds.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x124, 0xf>(v);  // row_ror:4
+  v += dpp_f64<0x128, 0xf>(v);  // row_ror:8
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 // Deterministic sum of `nv` values per thread over the block; result valid in thread 0.

@@ -121,17 +121,24 @@ __global__ void k_kweights(EwCols cols, double inv4a2, double* __restrict__ kw) 
   }
 }
 
-// F_i = scale q_i sum_k Im(conj(S_k) e^{i k.r_i}) w_k k, one thread per particle.
-__global__ void __launch_bounds__(kET) k_recip_force(const double* __restrict__ pos, const double* __restrict__ q,
-                                                     int n, EwCols cols, const double2* __restrict__ A, double scale,
-                                                     double* __restrict__ F, int accumulate) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
+// F_i = scale q_i sum_k Im(conj(S_k) e^{i k.r_i}) w_k k. A block holds 64 particles, one
+// per lane, and kRP waves that take every kRP-th column each, so that kRP waves per SIMD
+// hide the latency of the per-column rotation chains; the waves' partial forces are
+// added in wave order through the LDS.
+constexpr int kRP = 4;
+__global__ void __launch_bounds__(64 * kRP) k_recip_force(const double* __restrict__ pos, const double* __restrict__ q,
+                                                          int n, EwCols cols, const double2* __restrict__ A,
+                                                          double scale, double* __restrict__ F, int accumulate) {
+  __shared__ double red[kRP - 1][3][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 64 + lane;
+  const bool on = i < n;
+  const double x = on ? pos[3 * (size_t)i] : 0.0, y = on ? pos[3 * (size_t)i + 1] : 0.0,
+               z = on ? pos[3 * (size_t)i + 2] : 0.0;
   double ezs, ezc;
   sincos(cols.tw[2] * z, &ezs, &ezc);
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int c = 0; c < cols.ncol; ++c) {
+  for (int c = w; c < cols.ncol; c += kRP) {
     const int nx = cols.col[4 * c], ny = cols.col[4 * c + 1], lo = cols.col[4 * c + 2], hi = cols.col[4 * c + 3];
     const double kx = cols.tw[0] * nx, ky = cols.tw[1] * ny;
     double er, ei;
@@ -152,6 +159,19 @@ __global__ void __launch_bounds__(kET) k_recip_force(const double* __restrict__ 
     fx = fma(kx, g, fx);
     fy = fma(ky, g, fy);
     fz = fma(cols.tw[2], gz, fz);
+  }
+  if (w > 0) {
+    red[w - 1][0][lane] = fx;
+    red[w - 1][1][lane] = fy;
+    red[w - 1][2][lane] = fz;
+  }
+  __syncthreads();
+  if (w != 0 || !on) return;
+#pragma unroll
+  for (int r = 0; r < kRP - 1; ++r) {
+    fx += red[r][0][lane];
+    fy += red[r][1][lane];
+    fz += red[r][2][lane];
   }
   const double s = scale * q[i];
   if (accumulate) {
@@ -247,9 +267,9 @@ int accu_ewald_recip(const double* pos, const double* q, int n, const double* bo
                      Ud);
   // U_rec = 2 * (2 pi / V) * sum_half w |S|^2, times the Coulomb constant
   if (U) hipLaunchKernelGGL(k_scale1, dim3(1), dim3(64), 0, st, U, 4.0 * M_PI / V * k);
-  const int nbf = (n + kET - 1) / kET;
-  hipLaunchKernelGGL(k_recip_force, dim3(nbf), dim3(kET), 0, st, pos, q, n, ec, s.A, 2.0 * 4.0 * M_PI / V * k, F,
-                     accumulate);
+  const int nbf = (n + 63) / 64;
+  hipLaunchKernelGGL(k_recip_force, dim3(nbf), dim3(64 * kRP), 0, st, pos, q, n, ec, s.A, 2.0 * 4.0 * M_PI / V * k,
+                     F, accumulate);
   LAUNCH_CHECK();
   return ACCU_OK;
 }
