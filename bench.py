@@ -264,7 +264,8 @@ def gravity_plummer(n=262144, seed=11):
 def learned_throughput(reps=5):
     """The learned correction on the 65,536-ion NaCl workload, on the GPU: evaluations/s of
     DSF + the shipped network against Ewald (real-space error 1e-6), each DSF and
-    correction call building its neighbour list afresh (a rebuild every evaluation), and
+    correction call rebuilding its neighbour list (a rebuild every evaluation, into the
+    instance's kept device buffers), and
     their RMS relative force error against Ewald on a snapshot of an independent melt
     (Langevin 3000 K then 1200 K, seed 77) that the network never saw, beside DSF alone."""
     import torch
@@ -276,12 +277,16 @@ def learned_throughput(reps=5):
     ewald = train_nacl.ewald_1e6()
     corr = learned.load_correction()
 
-    def dsf_fresh(sys_):  # a new instance: its list is built inside the call
-        return dp.DampedShiftedCoulomb(rcut=rc, alpha=train_nacl.ALPHA_DSF, precision="mixed").compute(sys_)
+    dsf = dp.DampedShiftedCoulomb(rcut=rc, alpha=train_nacl.ALPHA_DSF, precision="mixed")
+    ldsf = learned.LearnedDSF(rcut=rc, alpha=train_nacl.ALPHA_DSF, precision="mixed", correction=corr)
 
-    def learned_fresh(sys_):  # DSF + correction over one list, built inside the call
-        return learned.LearnedDSF(rcut=rc, alpha=train_nacl.ALPHA_DSF, precision="mixed",
-                                  correction=corr).compute(sys_)
+    def dsf_fresh(sys_):  # the list is rebuilt inside every call
+        dsf.invalidate_list()
+        return dsf.compute(sys_)
+
+    def learned_fresh(sys_):  # DSF + correction over one list, rebuilt inside every call
+        ldsf.invalidate_list()
+        return ldsf.compute(sys_)
 
     def timed(fn):
         fn(s)

@@ -1,4 +1,4 @@
-// Cell binning, stable counting sort by cell, and the Verlet (ELL) neighbour list.
+// Cell binning, a stable sort by cell (rocPRIM radix sort), and the Verlet (ELL) neighbour list.
 //
 // Follows accusim/forces.py:279-321 (cell_list_pairs) and 78-98 (VerletList):
 //   cell = clamp(trunc(((pos - lo) / span) * ncell), 0, ncell - 1), flat C-order index,
@@ -111,18 +111,19 @@ __global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int 
         dy = pos[(size_t)stride_pos * j + 1] - yi;
         dz = pos[(size_t)stride_pos * j + 2] - zi;
       }
+      double r2;
       if (g.periodic) {
-        if (exact) {
-          dx = minimg_exact(dx, g.box[0]);
-          dy = minimg_exact(dy, g.box[1]);
-          dz = minimg_exact(dz, g.box[2]);
-        } else {
-          dx = minimg_fast(dx, g.box[0], inv[0]);
-          dy = minimg_fast(dy, g.box[1], inv[1]);
-          dz = minimg_fast(dz, g.box[2], inv[2]);
-        }
+        // reciprocal-multiply minimum image first: its r2 is within a few ulps of the
+        // exact one (it picks another image only at a rounding tie, |d| = L/2, where both
+        // images are L/2 away), so only a candidate within 1e-10 of rc2 needs the exact
+        // expression with the true division to decide membership bit for bit
+        r2 = r2_np(minimg_fast(dx, g.box[0], inv[0]), minimg_fast(dy, g.box[1], inv[1]),
+                   minimg_fast(dz, g.box[2], inv[2]));
+        if (exact && fabs(r2 - rc2) <= 1e-10 * rc2)
+          r2 = r2_np(minimg_exact(dx, g.box[0]), minimg_exact(dy, g.box[1]), minimg_exact(dz, g.box[2]));
+      } else {
+        r2 = r2_np(dx, dy, dz);
       }
-      const double r2 = r2_np(dx, dy, dz);
       if (r2 < rc2) {
         if (cnt < max_nbrs) nbr[(size_t)cnt * stride + t] = j;
         ++cnt;
@@ -133,6 +134,126 @@ __global__ void k_build_ell(const double* __restrict__ pos, int stride_pos, int 
   // maximum goes to maxcount so the host can detect an overflow and regrow
   nn[t] = cnt < max_nbrs ? cnt : max_nbrs;
   atomicMax(maxcount, cnt);
+}
+
+// The same list built one home cell per block: the candidates of the cell's 27-cell stencil
+// (in the order above: stencil cell, then slot) are staged into the LDS in chunks, and
+// every atom of the cell (one per thread) tests them there. All lanes of a wave read the
+// same candidate, so the LDS reads broadcast, and the global gathers of a candidate happen
+// once per cell instead of once per atom. Entries, their order and the counts are those of
+// k_build_ell.
+constexpr int kEB = 256;   // threads per block
+constexpr int kECh = 1024;  // candidates per LDS chunk (28 KiB)
+__global__ void __launch_bounds__(kEB) k_build_ell_cell(const double* __restrict__ pos, int stride_pos, CellGeo g,
+                                                        double rc2, const int* __restrict__ start,
+                                                        const int* __restrict__ sorted, int* __restrict__ nn,
+                                                        int* __restrict__ nbr, int stride, int max_nbrs,
+                                                        int* __restrict__ maxcount, int exact) {
+  __shared__ double lx[kECh], ly[kECh], lz[kECh];
+  __shared__ int lid[kECh];
+  __shared__ int cf[27], cst[28];
+  __shared__ int ncell;
+  const int f = blockIdx.x;
+  const int h0 = start[f], h1 = start[f + 1];
+  if (h0 == h1) return;
+  if (threadIdx.x == 0) {  // the stencil, de-duplicated and clipped as in k_build_ell
+    int ci[3];
+    unflat(f, g.nc, ci);
+    int m = 0, tot = 0;
+    for (int o = 0; o < 27; ++o) {
+      int c[3] = {ci[0] + o / 9 - 1, ci[1] + (o / 3) % 3 - 1, ci[2] + o % 3 - 1};
+      bool ok = true;
+      for (int d = 0; d < 3; ++d) {
+        if (g.periodic) c[d] = ((c[d] % g.nc[d]) + g.nc[d]) % g.nc[d];
+        else if (c[d] < 0 || c[d] >= g.nc[d]) ok = false;
+      }
+      if (!ok) continue;
+      const int q = (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2];
+      bool dup = false;
+      if (g.dedup)
+        for (int k = 0; k < m; ++k) dup |= (cf[k] == q);
+      if (dup) continue;
+      cf[m] = q;
+      cst[m] = tot;
+      tot += start[q + 1] - start[q];
+      ++m;
+    }
+    cst[m] = tot;
+    ncell = m;
+  }
+  __syncthreads();
+  const int m = ncell, total = cst[m];
+  const double inv[3] = {1.0 / g.box[0], 1.0 / g.box[1], 1.0 / g.box[2]};
+  for (int a0 = h0; a0 < h1; a0 += blockDim.x) {
+    const int t = a0 + (int)threadIdx.x;
+    const bool on = t < h1;
+    const int i = on ? sorted[t] : -1;
+    const double xi = on ? pos[(size_t)stride_pos * i] : 0.0, yi = on ? pos[(size_t)stride_pos * i + 1] : 0.0,
+                 zi = on ? pos[(size_t)stride_pos * i + 2] : 0.0;
+    int cnt = 0;
+    for (int base = 0; base < total; base += kECh) {
+      const int len = min(kECh, total - base);
+      __syncthreads();  // the previous chunk is no longer read
+      for (int k = threadIdx.x; k < len; k += blockDim.x) {
+        const int q = base + k;
+        int lo = 0, hi = m - 1;  // the stencil cell r with cst[r] <= q < cst[r + 1]
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (cst[mid] <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        const int j = sorted[start[cf[lo]] + (q - cst[lo])];
+        lx[k] = pos[(size_t)stride_pos * j];
+        ly[k] = pos[(size_t)stride_pos * j + 1];
+        lz[k] = pos[(size_t)stride_pos * j + 2];
+        lid[k] = j;
+      }
+      __syncthreads();
+      if (!on) continue;
+      auto test = [&](int j, double xj, double yj, double zj) {
+        if (j == i) return;
+        double dx, dy, dz;
+        {
+#pragma clang fp contract(off)
+          dx = xj - xi;
+          dy = yj - yi;
+          dz = zj - zi;
+        }
+        double r2;
+        if (g.periodic) {  // as in k_build_ell: the fast image, the exact one near rc2
+          r2 = r2_np(minimg_fast(dx, g.box[0], inv[0]), minimg_fast(dy, g.box[1], inv[1]),
+                     minimg_fast(dz, g.box[2], inv[2]));
+          if (exact && fabs(r2 - rc2) <= 1e-10 * rc2)
+            r2 = r2_np(minimg_exact(dx, g.box[0]), minimg_exact(dy, g.box[1]), minimg_exact(dz, g.box[2]));
+        } else {
+          r2 = r2_np(dx, dy, dz);
+        }
+        if (r2 < rc2) {
+          if (cnt < max_nbrs) nbr[(size_t)cnt * stride + t] = j;
+          ++cnt;
+        }
+      };
+      int k = 0;
+      for (; k + 4 <= len; k += 4) {  // four candidates' LDS reads in flight together
+        double X[4], Y[4], Z[4];
+        int J[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          X[u] = lx[k + u];
+          Y[u] = ly[k + u];
+          Z[u] = lz[k + u];
+          J[u] = lid[k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) test(J[u], X[u], Y[u], Z[u]);
+      }
+      for (; k < len; ++k) test(lid[k], lx[k], ly[k], lz[k]);
+    }
+    if (on) {
+      nn[t] = cnt < max_nbrs ? cnt : max_nbrs;
+      atomicMax(maxcount, cnt);
+    }
+  }
 }
 
 __global__ void k_count_half(int n, const int* __restrict__ sorted, const int* __restrict__ nn,
@@ -278,9 +399,14 @@ int launch_bin_sort(NList& h, const double* pos, int stride_pos, int n, const Ce
 
 int launch_build_ell(NList& h, const double* pos, int stride_pos, int n, const CellGeo& g, double rc2,
                      const int* sorted, hipStream_t st, int exact) {
-  const int T = 256;
-  hipLaunchKernelGGL(k_build_ell, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, rc2, h.cell, h.start,
-                     sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, exact);
+  if (sorted && h.ncells < (1LL << 31)) {  // one block per home cell over the cell-sorted atoms
+    hipLaunchKernelGGL(k_build_ell_cell, dim3((unsigned)h.ncells), dim3(kEB), 0, st, pos, stride_pos, g, rc2, h.start,
+                       sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, exact);
+  } else {
+    const int T = 256;
+    hipLaunchKernelGGL(k_build_ell, dim3((n + T - 1) / T), dim3(T), 0, st, pos, stride_pos, n, g, rc2, h.cell,
+                       h.start, sorted, h.nn, h.nbr, h.stride, h.max_nbrs, h.dev_scalars, exact);
+  }
   LAUNCH_CHECK();
   return ACCU_OK;
 }

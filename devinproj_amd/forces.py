@@ -169,6 +169,10 @@ class VerletList:
         self.rebuilds += 1
         return True
 
+    def invalidate(self):
+        """The next update rebuilds the list whatever the displacements."""
+        self._state = None
+
     def pairs(self, s, rc, skin):
         pos_d = self._ref.upload("pos", np.asarray(s.pos, float), L.torch().float64)
         self.update(s, pos_d, rc, skin)
@@ -212,6 +216,11 @@ class LennardJones(_DeviceForce):
     @property
     def rebuilds(self):
         return self._vl.rebuilds
+
+    def invalidate_list(self):
+        """Rebuild the neighbour list at the next compute (a fresh list, with its device
+        buffers kept)."""
+        self._vl.invalidate()
 
     def _tables(self, s):
         # forces.py:144-155
@@ -286,6 +295,11 @@ class DampedShiftedCoulomb(_DeviceForce):
     @property
     def rebuilds(self):
         return self._vl.rebuilds
+
+    def invalidate_list(self):
+        """Rebuild the neighbour list at the next compute (a fresh list, with its device
+        buffers kept)."""
+        self._vl.invalidate()
 
     def device_forces(self, s):
         """The forces on the device: (F, U, positions) device buffers and the neighbour
