@@ -115,6 +115,19 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Sums over the 4 lanes of each quad (every lane of the quad gets it) in a fixed order, by
+// DPP quad permutes. All 4 lanes of the quad must be active.
+__device__ __forceinline__ double quad_sum(double v) {
+  v += dpp_f64<0xB1, 0xf>(v);  // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E, 0xf>(v);  // quad_perm [2, 3, 0, 1]
+  return v;
+}
+__device__ __forceinline__ float quad_sumf(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xf, 0xf, false));
+  return v;
+}
+
 // Sum over the 64 lanes of a wave, in a fixed order, returned to every lane. All lanes
 // must be active. Cross-lane moves are DPP (quad permutes, row rotations, row
 // broadcasts), so the sum takes no LDS instructions: quads, then 16-lane rows, then

@@ -59,27 +59,35 @@ __device__ __forceinline__ float dsilu(float z) {
 __global__ void k_mlp_coord(const double* __restrict__ pos, int ps, int n, const int* __restrict__ sorted,
                             const int* __restrict__ nn, const int* __restrict__ nbr, int stride, Box box, MLPGeom g,
                             float* __restrict__ coord) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = sorted ? sorted[t] : t;
-  const double xi = pos[(size_t)ps * i], yi = pos[(size_t)ps * i + 1], zi = pos[(size_t)ps * i + 2];
+  // a quad of lanes per atom, lane s taking list entries s, s + 4, ... (four times the
+  // waves of one thread per atom, for the latency of the gathers); quad sums in a fixed order
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = gt >> 2, sub = gt & 3;
+  const bool on = t < n;
   float c = 0.f;
-  const float w = g.r_off - g.r_on;
-  for (int k = 0; k < nn[t]; ++k) {
-    const int j = nbr[(size_t)k * stride + t];
-    double dx = pos[(size_t)ps * j] - xi, dy = pos[(size_t)ps * j + 1] - yi, dz = pos[(size_t)ps * j + 2] - zi;
-    if (box.periodic) {
-      dx = minimg_fast(dx, box.L[0], box.inv[0]);
-      dy = minimg_fast(dy, box.L[1], box.inv[1]);
-      dz = minimg_fast(dz, box.L[2], box.inv[2]);
-    }
-    const float r = (float)sqrt(dx * dx + dy * dy + dz * dz);
-    if (r < g.r_off) {
-      const float xr = fminf(fmaxf((r - g.r_on) / w, 0.f), 1.f);
-      c += 0.5f * (1.f + cosf(3.14159265358979f * xr));
+  int i = 0;
+  if (on) {
+    i = sorted ? sorted[t] : t;
+    const double xi = pos[(size_t)ps * i], yi = pos[(size_t)ps * i + 1], zi = pos[(size_t)ps * i + 2];
+    const float w = g.r_off - g.r_on;
+    const int cnt = nn[t];
+    for (int k = sub; k < cnt; k += 4) {
+      const int j = nbr[(size_t)k * stride + t];
+      double dx = pos[(size_t)ps * j] - xi, dy = pos[(size_t)ps * j + 1] - yi, dz = pos[(size_t)ps * j + 2] - zi;
+      if (box.periodic) {
+        dx = minimg_fast(dx, box.L[0], box.inv[0]);
+        dy = minimg_fast(dy, box.L[1], box.inv[1]);
+        dz = minimg_fast(dz, box.L[2], box.inv[2]);
+      }
+      const float r = (float)sqrt(dx * dx + dy * dy + dz * dz);
+      if (r < g.r_off) {
+        const float xr = fminf(fmaxf((r - g.r_on) / w, 0.f), 1.f);
+        c += 0.5f * (1.f + cosf(3.14159265358979f * xr));
+      }
     }
   }
-  coord[i] = c;
+  c = quad_sumf(c);
+  if (on && sub == 0) coord[i] = c;
 }
 
 // Parameters staged in LDS (float32) and the per-lane W2 operands (bf16).
@@ -643,8 +651,8 @@ int accu_mlp_prepare(accu_mlp* h, accu_nlist* nlh, const double* pos, const doub
   int rc = m.ensure(n);
   if (rc) return rc;
   hipLaunchKernelGGL(k_to_float, dim3((n + 255) / 256), dim3(256), 0, st, q, qscale, n, m.q);
-  hipLaunchKernelGGL(k_mlp_coord, dim3((n + 255) / 256), dim3(256), 0, st, pos, 3, n, nl.sorted, nl.nn, nl.nbr,
-                     nl.stride, box_of(box, periodic), m.g, m.coord);
+  hipLaunchKernelGGL(k_mlp_coord, dim3((int)((4LL * n + 255) / 256)), dim3(256), 0, st, pos, 3, n, nl.sorted, nl.nn,
+                     nl.nbr, nl.stride, box_of(box, periodic), m.g, m.coord);
   LAUNCH_CHECK();
   return ACCU_OK;
 }

@@ -1,7 +1,11 @@
 // Pair forces over the Verlet list (N >= pairlist_threshold) or all pairs (small N):
 // Lennard-Jones, damped shifted force (DSF) Coulomb, Ewald real space.
-// Thread = one atom (its sorted slot), full neighbour list: each thread sums its own
-// force in float64 and never scatters, so results do not depend on scheduling.
+// List kernels: a quad of lanes per atom (its sorted slot), lane s of the quad taking list
+// entries s, s + 4, ...; the full list means each atom's force is summed by its own quad in
+// float64 (the quad's sums combined in a fixed order) and never scattered, so results do
+// not depend on scheduling. Four lanes per atom give four times the waves of one thread
+// per atom, which the latency of the neighbour gathers needs. Dense kernels: one thread
+// per atom.
 #include "common.h"
 #include "nlist.h"
 #include "pairkern.h"
@@ -29,39 +33,29 @@ __global__ void __launch_bounds__(kT) k_lj_list(LJParams prm, const double* __re
   it.stage(prm, lds);
   __syncthreads();
   const int ns = prm.ns;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  double U[1] = {0.0};
-  if (t < n) {
-    const int i = sorted ? sorted[t] : t;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int t = g >> 2, sub = g & 3;
+  const bool on = t < n;
+  double fx = 0, fy = 0, fz = 0, u = 0;
+  int i = 0;
+  if (on) {
+    i = sorted ? sorted[t] : t;
     const double xi = pos[(size_t)ps * i], yi = pos[(size_t)ps * i + 1], zi = pos[(size_t)ps * i + 2];
     const int row = sp[i] * ns;
     const int cnt = nn[t];
-    double fx = 0, fy = 0, fz = 0, u = 0;
-    if constexpr (sizeof(P) == 8) {
-      for (int k = 0; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * stride + t];
-        double dx = pos[(size_t)ps * j] - xi, dy = pos[(size_t)ps * j + 1] - yi, dz = pos[(size_t)ps * j + 2] - zi;
-        if (box.periodic) {
-          dx = minimg_fast(dx, box.L[0], box.inv[0]);
-          dy = minimg_fast(dy, box.L[1], box.inv[1]);
-          dz = minimg_fast(dz, box.L[2], box.inv[2]);
-        }
-        const int pj = row + sp[j];
-        lj_pair64(dx, dy, dz, r2_np(dx, dy, dz), it.eps[pj], it.sig2[pj], it.rc2[pj], it.shift, fx, fy, fz, u);
+    for (int k = sub; k < cnt; k += 4) {
+      const int j = nbr[(size_t)k * stride + t];
+      double dx = pos[(size_t)ps * j] - xi, dy = pos[(size_t)ps * j + 1] - yi, dz = pos[(size_t)ps * j + 2] - zi;
+      if (box.periodic) {
+        dx = minimg_fast(dx, box.L[0], box.inv[0]);
+        dy = minimg_fast(dy, box.L[1], box.inv[1]);
+        dz = minimg_fast(dz, box.L[2], box.inv[2]);
       }
-    } else {
-      // float32 pair math on the float64 minimum-image displacement; float64 sums
-      float ux = 0.f;
-      for (int k = 0; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * stride + t];
-        double dx = pos[(size_t)ps * j] - xi, dy = pos[(size_t)ps * j + 1] - yi, dz = pos[(size_t)ps * j + 2] - zi;
-        if (box.periodic) {
-          dx = minimg_fast(dx, box.L[0], box.inv[0]);
-          dy = minimg_fast(dy, box.L[1], box.inv[1]);
-          dz = minimg_fast(dz, box.L[2], box.inv[2]);
-        }
+      const int pj = row + sp[j];
+      if constexpr (sizeof(P) == 8) {
+        lj_pair64(dx, dy, dz, r2_np(dx, dy, dz), it.eps[pj], it.sig2[pj], it.rc2[pj], it.shift, fx, fy, fz, u);
+      } else {  // float32 pair math on the float64 minimum-image displacement; float64 sums
         const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
-        const int pj = row + sp[j];
         float gx = 0.f, gy = 0.f, gz = 0.f, gu = 0.f;
         lj_pair32(fdx, fdy, fdz, r2_npf(fdx, fdy, fdz), it.eps[pj], it.sig2[pj], it.rc2[pj], it.shift, gx, gy, gz,
                   gu);
@@ -70,8 +64,14 @@ __global__ void __launch_bounds__(kT) k_lj_list(LJParams prm, const double* __re
         fz += (double)gz;
         u += (double)gu;
       }
-      (void)ux;
     }
+  }
+  fx = quad_sum(fx);
+  fy = quad_sum(fy);
+  fz = quad_sum(fz);
+  u = quad_sum(u);
+  double U[1] = {0.0};
+  if (on && sub == 0) {
     F[3 * (size_t)i] = fx;
     F[3 * (size_t)i + 1] = fy;
     F[3 * (size_t)i + 2] = fz;
@@ -124,15 +124,17 @@ __global__ void __launch_bounds__(kT) k_coul_list(CoulParams p, const double* __
                                                   Box box, double* __restrict__ F, double* partials,
                                                   unsigned* counter, double* Uout) {
   __shared__ double red[16];
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  double U[1] = {0.0};
-  if (t < n) {
-    const int i = sorted ? sorted[t] : t;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int t = g >> 2, sub = g & 3;
+  const bool on = t < n;
+  double fx = 0, fy = 0, fz = 0, u = 0, qi = 0;
+  int i = 0;
+  if (on) {
+    i = sorted ? sorted[t] : t;
     const double xi = pos[(size_t)ps * i], yi = pos[(size_t)ps * i + 1], zi = pos[(size_t)ps * i + 2];
-    const double qi = q[i];
+    qi = q[i];
     const int cnt = nn[t];
-    double fx = 0, fy = 0, fz = 0, u = 0;
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = sub; k < cnt; k += 4) {
       const int j = nbr[(size_t)k * stride + t];
       if (p.nex && excluded(p, i, j, n)) continue;
       double dx = pos[(size_t)ps * j] - xi, dy = pos[(size_t)ps * j + 1] - yi, dz = pos[(size_t)ps * j + 2] - zi;
@@ -155,6 +157,13 @@ __global__ void __launch_bounds__(kT) k_coul_list(CoulParams p, const double* __
         u += (double)gu;
       }
     }
+  }
+  fx = quad_sum(fx);
+  fy = quad_sum(fy);
+  fz = quad_sum(fz);
+  u = quad_sum(u);
+  double U[1] = {0.0};
+  if (on && sub == 0) {
     u -= qi * qi * p.self;
     F[3 * (size_t)i] = p.k * fx;
     F[3 * (size_t)i + 1] = p.k * fy;
@@ -234,17 +243,17 @@ int accu_lj_forces(accu_nlist* hh, const double* pos, const int* sp, int n, int 
   hipStream_t st = (hipStream_t)stream;
   if (n <= 0) return ACCU_OK;
   Workspace& ws = default_workspace();
-  const int nb = (n + kT - 1) / kT;
-  if ((rc = ws.ensure((size_t)nb * 2))) return rc;
+  const int nb = (n + kT - 1) / kT, nbl = (int)((4LL * n + kT - 1) / kT);  // dense / list (a quad per atom)
+  if ((rc = ws.ensure((size_t)nbl * 2))) return rc;
   Box b = make_box(box, periodic);
   if (hh) {
     NList& h = *reinterpret_cast<NList*>(hh);
     ARG_CHECK(h.n == n, "neighbour list was built for %d atoms, got %d", h.n, n);
     if (precision == ACCU_FP64)
-      hipLaunchKernelGGL(k_lj_list<double>, dim3(nb), dim3(kT), 0, st, prm, pos, 3, sp, n, h.sorted, h.nn, h.nbr,
+      hipLaunchKernelGGL(k_lj_list<double>, dim3(nbl), dim3(kT), 0, st, prm, pos, 3, sp, n, h.sorted, h.nn, h.nbr,
                          h.stride, b, F, ws.partials, ws.counter, U);
     else
-      hipLaunchKernelGGL(k_lj_list<float>, dim3(nb), dim3(kT), 0, st, prm, pos, 3, sp, n, h.sorted, h.nn, h.nbr,
+      hipLaunchKernelGGL(k_lj_list<float>, dim3(nbl), dim3(kT), 0, st, prm, pos, 3, sp, n, h.sorted, h.nn, h.nbr,
                          h.stride, b, F, ws.partials, ws.counter, U);
   } else {
     hipLaunchKernelGGL(k_lj_dense, dim3(nb), dim3(kT), 0, st, prm, pos, sp, n, b, F, ws.partials, ws.counter, U);
@@ -275,18 +284,18 @@ int accu_coulomb_forces(accu_nlist* hh, const double* pos, const double* q, int 
   p.nex = nex;
   p.ex = ex_keys;
   Workspace& ws = default_workspace();
-  const int nb = (n + kT - 1) / kT;
-  int rc = ws.ensure((size_t)nb * 2);
+  const int nb = (n + kT - 1) / kT, nbl = (int)((4LL * n + kT - 1) / kT);  // dense / list (a quad per atom)
+  int rc = ws.ensure((size_t)nbl * 2);
   if (rc) return rc;
   Box b = make_box(box, periodic);
   if (hh) {
     NList& h = *reinterpret_cast<NList*>(hh);
     ARG_CHECK(h.n == n, "neighbour list was built for %d atoms, got %d", h.n, n);
     if (precision == ACCU_FP64)
-      hipLaunchKernelGGL(k_coul_list<double>, dim3(nb), dim3(kT), 0, st, p, pos, 3, q, n, h.sorted, h.nn, h.nbr,
+      hipLaunchKernelGGL(k_coul_list<double>, dim3(nbl), dim3(kT), 0, st, p, pos, 3, q, n, h.sorted, h.nn, h.nbr,
                          h.stride, b, F, ws.partials, ws.counter, U);
     else
-      hipLaunchKernelGGL(k_coul_list<float>, dim3(nb), dim3(kT), 0, st, p, pos, 3, q, n, h.sorted, h.nn, h.nbr,
+      hipLaunchKernelGGL(k_coul_list<float>, dim3(nbl), dim3(kT), 0, st, p, pos, 3, q, n, h.sorted, h.nn, h.nbr,
                          h.stride, b, F, ws.partials, ws.counter, U);
   } else {
     hipLaunchKernelGGL(k_coul_dense, dim3(nb), dim3(kT), 0, st, p, pos, q, n, b, F, ws.partials, ws.counter, U);
