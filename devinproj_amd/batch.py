@@ -200,6 +200,9 @@ class BrickEngine:
         err = np.zeros(2, np.int32)
         L.call("accu_bmd_get_constraints", self._h, L.ptr(out), L.ptr(err))
         code = int(err[0])
+        # the failing cluster, indexed as _set_constraints listed them (integrators.
+        # constraint_clusters order, system after system); None while nothing failed
+        self.failed_cluster = int(err[1]) if code else None
         if code == 1:
             raise RuntimeError("SHAKE did not converge: reduce dt")
         if code == 2:
@@ -227,6 +230,10 @@ class BrickEngine:
         self.steps += steps
         if sync:
             self.sync()
+            if isinstance(self.integrator, Rattle):
+                # a failed SHAKE or RATTLE raises here, at the end of the run that hit it
+                # (the device keeps stepping the other clusters until the run ends)
+                self.constraint_state()
         return self
 
     def sync(self):
@@ -244,8 +251,8 @@ class BrickEngine:
         L.call("accu_bmd_get", self._h, L.ptr(pos), L.ptr(vel), L.ptr(F), L.ptr(log), self.log_cap, L.ptr(stats),
                L.ptr(info))
         khz = max(int(info[12]), 1)
-        ends = [1e3 * int(t) / khz for t in info[5:10]]
-        names = ("binning", "block_sums", "scan", "grouping", "order_and_gather")
+        ends = [1e3 * int(info[5 + k]) / khz for k in (0, 1, 2, 4)]  # phase ends after the rebuild's start
+        names = ("binning", "scan", "grouping", "order_and_gather")  # grouping: 0 unless a bucket overflowed
         self.info = {"max_neighbours": int(info[0]), "capacity": int(info[1]), "region_capacity": int(info[2]),
                      "list_entries": int(info[4]),
                      "rebuild_phase_us": {n: round(e - (ends[k - 1] if k else 0.0), 1)

@@ -315,14 +315,18 @@ struct RB {
   int* count;   // per cell; all zero between rebuilds
   int* start;   // [ncells + 1]
   int* cursor;  // [ncells + 1]
-  int* tmp;     // atoms grouped by cell
+  int* tmp;     // atoms grouped by cell (when a bucket overflowed)
   int* slot;    // each atom's arrival slot in its cell
+  int* bucket;  // [ncells][bcap]: atoms of each cell in arrival order, written by the binning
+  int bcap;     // bucket capacity per cell (0: no buckets, group through tmp)
   int* bsum;    // [grid]
+  unsigned long long* look;  // [grid] the scan's per-block sums, tagged with the barrier generation
   int* nn;
   uint4* nbr;   // groups of 8 u16 local indices: group g of atom a at [g * stride + a]
   int stride, ngroups, cap;
-  int* ints;    // [0] max neighbours, [1] region overflow, [2] barrier timeout, [3] blocks seen then
-  long long* tphase;  // [0] start, [1..5] phase ends of the last rebuild (device wall clock)
+  int* ints;    // [0] max neighbours, [1] region overflow, [2] barrier timeout, [3] blocks seen then,
+                // [4] a cell outgrew its bucket in this rebuild
+  long long* tphase;  // [0] start; ends of binning, scan, grouping ([4] = [3]), gather (device wall clock)
   unsigned* bar;
   int* flags;   // [S] per system + [S] any, set by the fused step
   int* bflags;  // the same, as handed to the list launch
@@ -588,7 +592,12 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
         int base = 0;
         if (lane == leader) base = atomicAdd(&p.count[g], __popcll(same));
         base = __shfl(base, leader, 64);
-        if (gid == g) p.slot[a] = base + __popcll(same & ((1ull << lane) - 1ull));
+        if (gid == g) {
+          const int sl = base + __popcll(same & ((1ull << lane) - 1ull));
+          p.slot[a] = sl;
+          if (sl < p.bcap) p.bucket[(size_t)g * p.bcap + sl] = a;
+          else if (p.bcap) atomicOr(&p.ints[4], 1);
+        }
         todo &= ~same;
       }
     }
@@ -597,44 +606,77 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     if (!p.flags[p.cellsys[c]]) p.count[c] = p.start[c + 1] - p.start[c];
   if (!grid_sync(p.bar, p.ints + 2)) return;
   if (blockIdx.x == 0 && tid == 0) p.tphase[1] = wall_clock64();
-  // 2. exclusive scan of the counts over all cells (chunk per block), counts back to 0
+  // 2. exclusive scan of the counts to the cell starts, counts back to 0, in one phase: each
+  //    block sums its chunk of cells and publishes the sum in one 64-bit word tagged with
+  //    this phase's barrier generation (so no word needs clearing), adds the sums of the
+  //    blocks before it as they appear, then scans its chunk
   const int chunk = (p.ncells + gridDim.x - 1) / gridDim.x;
   const int c0 = min((int)blockIdx.x * chunk, p.ncells), c1 = min(c0 + chunk, p.ncells);
   {
+    const unsigned long long tag =
+        ((unsigned long long)(__hip_atomic_load(p.bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x7fffffffu) +
+         1ull) << 32;
     int v = 0;
     for (int c = c0 + tid; c < c1; c += bd) v += p.count[c];
     v = block_sum_int(v, wsum);
-    if (tid == 0) p.bsum[blockIdx.x] = v;
-  }
-  if (!grid_sync(p.bar, p.ints + 2)) return;
-  if (blockIdx.x == 0 && tid == 0) p.tphase[2] = wall_clock64();
-  {
-    int off = 0;
-    for (int k = tid; k < (int)blockIdx.x; k += bd) off += p.bsum[k];
-    off = block_sum_int(off, wsum);
+    if (tid == 0)
+      __hip_atomic_store(&p.look[blockIdx.x], tag | (unsigned)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ int s_off;
+    if (tid < 64) {
+      int off = 0;
+      for (int b0 = 0; b0 < (int)blockIdx.x; b0 += 64) {
+        const int b = b0 + tid;
+        if (b < (int)blockIdx.x) {
+          unsigned long long w = __hip_atomic_load(&p.look[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          int spins = 0;
+          for (; (w >> 32) != (tag >> 32); w = __hip_atomic_load(&p.look[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (++spins > kBarSpin) {  // reported like a barrier timeout
+              atomicOr(p.ints + 2, 1);
+              w = 0;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          off += (int)(unsigned)w;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o, 64);
+      if (tid == 0) s_off = off;
+    }
+    __syncthreads();
+    int off = s_off;
     for (int base = c0; base < c1; base += bd) {
       const int c = base + tid;
-      const int v = c < c1 ? p.count[c] : 0;
+      const int x = c < c1 ? p.count[c] : 0;
       int tot;
-      const int ex = block_excl_scan(v, wsum, tot);
+      const int ex = block_excl_scan(x, wsum, tot);
       if (c < c1) {
         p.start[c] = off + ex;
         p.count[c] = 0;
       }
       off += tot;
     }
-    if (c1 == p.ncells && tid == 0) p.start[p.ncells] = off;
+    if (c1 == p.ncells && c0 < c1 && tid == 0) p.start[p.ncells] = off;
   }
   if (p.bin_only) return;
   if (!grid_sync(p.bar, p.ints + 2)) return;
-  if (blockIdx.x == 0 && tid == 0) p.tphase[3] = wall_clock64();
-  // 3. atoms grouped by cell (arrival order inside a cell; fixed in 4)
-  for (int a = gtid; a < p.N; a += gsz) {
-    const int si = p.sysid[a];
-    if (p.flags[si]) p.tmp[p.start[p.st.cell[p.spar[si]][a]] + p.slot[a]] = a;
+  if (blockIdx.x == 0 && tid == 0) p.tphase[2] = wall_clock64();
+  // 3. only when some cell outgrew its bucket: atoms grouped by cell through their slots
+  const bool use_bucket =
+      p.bcap > 0 && __hip_atomic_load(&p.ints[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  if (!use_bucket) {
+    for (int a = gtid; a < p.N; a += gsz) {
+      const int si = p.sysid[a];
+      if (p.flags[si]) p.tmp[p.start[p.st.cell[p.spar[si]][a]] + p.slot[a]] = a;
+    }
+    if (!grid_sync(p.bar, p.ints + 2)) return;
   }
-  if (!grid_sync(p.bar, p.ints + 2)) return;
-  if (blockIdx.x == 0 && tid == 0) p.tphase[4] = wall_clock64();
+  if (blockIdx.x == 0 && tid == 0) p.tphase[3] = p.tphase[4] = wall_clock64();
+  // the atoms of cell c in arrival order
+  auto atoms_of = [&](int c) -> const int* {
+    return use_bucket ? p.bucket + (size_t)c * p.bcap : p.tmp + p.start[c];
+  };
   // 4. rank each cell's atoms by z (ties by original index) and gather each atom's state
   //    to its sorted slot in the other buffers; the list reference positions are the
   //    gathered positions. Cells stacked along z then form z-sorted runs of slots, which
@@ -667,7 +709,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       }
       if (!__any(len > 32)) {
         const bool mine = on && hl < len;
-        const int v = mine ? p.tmp[a + hl] : 0x7fffffff;
+        const int v = mine ? atoms_of(cm)[hl] : 0x7fffffff;
         const double zv = mine ? zkey(p, v, p.cellsys[cm]) : INFINITY;
         int r = 0;
         for (int x = 0; x < 32; ++x) {
@@ -682,10 +724,11 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
           if (c >= p.ncells || !p.flags[p.cellsys[c]]) continue;
           const int si = p.cellsys[c];
           const int ca = p.start[c], cl = p.start[c + 1] - ca;
+          const int* ac = atoms_of(c);
           for (int e0 = 0; e0 < cl; e0 += 64) {
             const int e = e0 + lane;
             const bool mine = e < cl;
-            const int v = mine ? p.tmp[ca + e] : 0x7fffffff;
+            const int v = mine ? ac[e] : 0x7fffffff;
             const double zv = mine ? zkey(p, v, si) : INFINITY;
             int r = 0;
             if (cl <= 64) {
@@ -696,7 +739,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
               }
             } else if (mine) {
               for (int x = 0; x < cl; ++x) {
-                const int vx = p.tmp[ca + x];
+                const int vx = ac[x];
                 const double zx = zkey(p, vx, si);
                 r += (zx < zv || (zx == zv && vx < v)) ? 1 : 0;
               }
@@ -727,6 +770,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     if (tid == 0) {
       p.bflags[p.S] = 1;
       p.flags[p.S] = 0;
+      p.ints[4] = 0;
     }
   }
 }
@@ -1723,6 +1767,8 @@ struct BMD {
   BState sta = {};                            // the rest of the per-atom state, buffers [0] / [1]
   int* spar = nullptr;                        // per system: which state buffer is current
   int *count = nullptr, *start = nullptr, *tmp = nullptr, *slot = nullptr;
+  int* bucket = nullptr;  // per-cell arrival buckets of the rebuild, bcap atoms each
+  int bcap = 0;
   int* nn = nullptr;
   uint4* nbr = nullptr;  // groups of 8 u16 local indices, group g of atom a at [g * stride + a]
   int stride = 0, max_nbrs = 0;  // max_nbrs: list capacity per atom, a multiple of 8
@@ -1736,11 +1782,13 @@ struct BMD {
   BCons cons = {};
   int ncl = 0, lds_k = 0;
   std::vector<int> hc_coff, hc_aoff, hc_la, hc_lb, hc_uatom, hc_csys;  // host copies of the clusters
+  std::vector<int> hc_user;  // cluster index as the caller passed it, per device cluster
   std::vector<double> hc_len;
   std::vector<void*> cbuf;
   int* bflags = nullptr;   // rebuild flags handed to the list launch
   unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
   int* bsum = nullptr;      // per-block sums of the rebuild scan
+  unsigned long long* look = nullptr;  // tagged per-block sums of the rebuild scan
   long long* tphase = nullptr;  // rebuild phase timestamps
   static constexpr int kMaxGrid = 1 << 16;
   int grid_rb = 0;  // resident grid of the rebuild launch
@@ -1769,8 +1817,8 @@ struct BMD {
   void release() {
     void* ps[] = {d_sys, d_bricks, d_reg, d_cellmap, d_cellsys, d_sysid, pos, pos_b, par, spar, sta.vel[0],
                   sta.vel[1], sta.F[0], sta.F[1], sta.ref[0], sta.ref[1], sta.perm[0], sta.perm[1], sta.cell[0],
-                  sta.cell[1], count, start, tmp, slot, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
-                  bar, bsum, tphase, snap, snap_count, snap_rec, ndrift, nh, sta.posf[0], sta.posf[1],
+                  sta.cell[1], count, start, tmp, slot, bucket, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
+                  bar, bsum, look, tphase, snap, snap_count, snap_rec, ndrift, nh, sta.posf[0], sta.posf[1],
                   (void*)sta.ccorner};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -1811,7 +1859,10 @@ struct BMD {
     p.start = start;
     p.tmp = tmp;
     p.slot = slot;
+    p.bucket = bucket;
+    p.bcap = bcap;
     p.bsum = bsum;
+    p.look = look;
     p.nn = nn;
     p.nbr = nbr;
     p.stride = stride;
@@ -2329,6 +2380,8 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   HIP_TRY(hipMalloc(&m.bar, 64));
   HIP_TRY(hipMemset(m.bar, 0, 64));
   HIP_TRY(hipMalloc(&m.bsum, BMD::kMaxGrid * sizeof(int)));
+  HIP_TRY(hipMalloc(&m.look, BMD::kMaxGrid * sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(m.look, 0, BMD::kMaxGrid * sizeof(unsigned long long)));
   HIP_TRY(hipMalloc(&m.tphase, 16 * sizeof(long long)));
   HIP_TRY(hipMemset(m.tphase, 0, 16 * sizeof(long long)));
   // null-stream fills complete before the first launch on the engine's non-blocking stream
@@ -2348,6 +2401,17 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
     HIP_TRY(hipStreamSynchronize(m.st));
     ARG_CHECK(e[2] == 0, "rebuild grid barrier timed out at the first binning: %d of %d blocks arrived", e[3],
               m.grid_rb);
+  }
+  {  // per-cell arrival buckets: twice the fullest cell of the first binning, plus 8; a
+     // rebuild in which some cell outgrows its bucket groups its atoms through tmp instead
+    int cmax = 0;
+    for (long long c = 0; c < m.ncells; ++c) cmax = std::max(cmax, hstart[c + 1] - hstart[c]);
+    const int bcap = (2 * cmax + 8 + 7) / 8 * 8;
+    const size_t bytes = (size_t)m.ncells * bcap * sizeof(int);
+    if (bytes <= std::max((size_t)64 << 20, (size_t)m.N * 16 * sizeof(int))) {
+      HIP_TRY(hipMalloc(&m.bucket, bytes));
+      m.bcap = bcap;
+    }
   }
   std::vector<BBrick> hb(m.nbricks);
   HIP_TRY(hipMemcpy(hb.data(), m.d_bricks, m.nbricks * sizeof(BBrick), hipMemcpyDeviceToHost));
@@ -2426,11 +2490,12 @@ static int order_clusters(BMD& m) {
   for (int c = 0; c < ncl; ++c) ord[c] = c;
   auto key = [&](int c) { return std::make_pair(m.hc_csys[c], islot[m.hc_uatom[m.hc_aoff[c]]]); };
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key(a) < key(b); });
-  std::vector<int> coff(1, 0), aoff(1, 0), la, lb, uatom, csys;
+  std::vector<int> coff(1, 0), aoff(1, 0), la, lb, uatom, csys, user;
   std::vector<double> len;
   std::vector<long long> moff;
   long long mtot = 0;
   for (int c : ord) {
+    user.push_back(m.hc_user[c]);
     for (int k = m.hc_coff[c]; k < m.hc_coff[c + 1]; ++k) {
       la.push_back(m.hc_la[k]);
       lb.push_back(m.hc_lb[k]);
@@ -2462,6 +2527,7 @@ static int order_clusters(BMD& m) {
   m.hc_len.swap(len);
   m.hc_uatom.swap(uatom);
   m.hc_csys.swap(csys);
+  m.hc_user.swap(user);
   return ACCU_OK;
 }
 
@@ -2593,6 +2659,8 @@ int accu_bmd_set_constraints(accu_bmd* h, int ncl, const int* coff, const int* a
   m.hc_len.assign(len, len + nb);
   m.hc_uatom.assign(uatom, uatom + na);
   m.hc_csys.assign(csys, csys + ncl);
+  m.hc_user.resize(ncl);
+  for (int c = 0; c < ncl; ++c) m.hc_user[c] = c;
   m.lds_k = kmax <= kLdsBonds ? kmax : 0;
   if (m.lds_k) {
     (void)hipFuncSetAttribute((const void*)k_b_shake<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2614,6 +2682,8 @@ int accu_bmd_get_constraints(accu_bmd* h, double* out, int* err) {
   HIP_TRY(hipMemcpyAsync(err, m.cons.cerr, 2 * sizeof(int), hipMemcpyDeviceToHost, m.st));
   HIP_TRY(hipStreamSynchronize(m.st));
   memcpy(out, &bits, sizeof(double));
+  // the kernels see the clusters in slot order (order_clusters); report the caller's index
+  if (err[0] != 0 && err[1] >= 0 && err[1] < (int)m.hc_user.size()) err[1] = m.hc_user[err[1]];
   return ACCU_OK;
 }
 

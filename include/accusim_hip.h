@@ -228,7 +228,8 @@ int accu_bmd_set_constraints(accu_bmd* h, int nclusters, const int* coff, const 
                              int max_iter);
 /* out_host[0]: the worst relative bond-length error after any SHAKE so far (Rattle.max_violation),
  * out_host[1]: the residual at a failure; err_host[0]: 0 ok, 1 SHAKE did not converge in
- * max_iter, 2 SHAKE stalled (float64 cannot reach tol), 3 singular RATTLE system; err_host[1]: the cluster */
+ * max_iter, 2 SHAKE stalled (float64 cannot reach tol), 3 singular RATTLE system; err_host[1]: the failing
+ * cluster's index c as passed to accu_bmd_set_constraints */
 int accu_bmd_get_constraints(accu_bmd* h, double* out_host, int* err_host);
 
 /* ---- analysis (accusim/analysis.py). pos / frames are device float64 arrays.

@@ -11,6 +11,7 @@ pytestmark = pytest.mark.gpu
 
 dp = pytest.importorskip("devinproj_amd")
 from devinproj_amd import batch  # noqa: E402
+from devinproj_amd.integrators import constraint_clusters  # noqa: E402
 
 LJ = {"A": (1.0, 0.8)}
 
@@ -92,22 +93,47 @@ def test_device_rattle_two_systems_one_arena_and_graph_replays():
 
 
 def test_device_rattle_failure_is_reported():
-    # a tolerance below float64's reach stalls the Newton iterations of every cluster
+    # a tolerance below float64's reach stalls the Newton iterations of every cluster: the
+    # run that hit it raises (a synchronised run checks the constraints), and so does fetch
     s0, cons = fluid("dimer", 64, 0.05, seed=2)
     rat = dp.Rattle(cons, tol=1e-30)
     eng = batch.BrickEngine([s0.copy()], [[dp.LennardJones(LJ, cutoff=2.5, skin=0.3)]], rat, 1e-3)
-    eng.run(2)
+    with pytest.raises(RuntimeError, match="stalled"):
+        eng.run(2)
     with pytest.raises(RuntimeError, match="stalled"):
         eng.fetch()
+    eng.run(1, sync=False)  # an unsynchronised run leaves the check to sync / fetch
+    eng.sync()
 
 
 def test_device_rattle_iteration_limit_is_reported():
     s0, cons = fluid("dimer", 64, 0.05, seed=2)
     rat = dp.Rattle(cons, tol=1e-12, max_iter=1)
     eng = batch.BrickEngine([s0.copy()], [[dp.LennardJones(LJ, cutoff=2.5, skin=0.3)]], rat, 1e-2)
-    eng.run(1)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        eng.run(1)
     with pytest.raises(RuntimeError, match="did not converge"):
         eng.fetch()
+
+
+def test_device_rattle_failure_names_the_callers_cluster():
+    # one dimer's bond given twice makes that cluster's SHAKE system singular (two equal
+    # rows), and only that cluster fails; the engine orders clusters by slot, the failure
+    # is reported by the cluster's index in the caller's (constraint_clusters) order
+    s0, cons = fluid("dimer", 64, 0.05, seed=2)
+    bad = 37
+    cons2 = list(cons) + [cons[bad]]
+    eng = batch.BrickEngine([s0.copy()], [[dp.LennardJones(LJ, cutoff=2.5, skin=0.3)]],
+                            dp.Rattle(cons2, tol=1e-10), 1e-3)
+    with pytest.raises(RuntimeError, match="stalled"):
+        eng.run(1)
+    where = {int(p): n for n, p in enumerate(s0.ids)}
+    ia = [where[a] for a, b, d in cons2]
+    ib = [where[b] for a, b, d in cons2]
+    groups = constraint_clusters(ia, ib)
+    expect = next(k for k, g in enumerate(groups) if len(cons2) - 1 in g)
+    assert eng.failed_cluster == expect
+    assert bad in groups[expect]
 
 
 def test_device_rattle_argument_checks():
