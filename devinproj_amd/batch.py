@@ -254,7 +254,7 @@ class BrickEngine:
         ends = [1e3 * int(info[5 + k]) / khz for k in (0, 1, 2, 4)]  # phase ends after the rebuild's start
         names = ("binning", "scan", "grouping", "order_and_gather")  # grouping: 0 unless a bucket overflowed
         self.info = {"max_neighbours": int(info[0]), "capacity": int(info[1]), "region_capacity": int(info[2]),
-                     "list_entries": int(info[4]),
+                     "list_entries": int(info[4]), "bucket_overflows": int(info[10]), "bucket_cap": int(info[11]),
                      "rebuild_phase_us": {n: round(e - (ends[k - 1] if k else 0.0), 1)
                                           for k, (n, e) in enumerate(zip(names, ends))}}
         self.rebuilds = stats[:, 0].copy()

@@ -325,7 +325,7 @@ struct RB {
   uint4* nbr;   // groups of 8 u16 local indices: group g of atom a at [g * stride + a]
   int stride, ngroups, cap;
   int* ints;    // [0] max neighbours, [1] region overflow, [2] barrier timeout, [3] blocks seen then,
-                // [4] a cell outgrew its bucket in this rebuild
+                // [4] a cell outgrew its bucket in this rebuild, [5] rebuilds in which one did
   long long* tphase;  // [0] start; ends of binning, scan, grouping ([4] = [3]), gather (device wall clock)
   unsigned* bar;
   int* flags;   // [S] per system + [S] any, set by the fused step
@@ -771,6 +771,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       p.bflags[p.S] = 1;
       p.flags[p.S] = 0;
       p.ints[4] = 0;
+      if (!use_bucket && p.bcap > 0) p.ints[5] += 1;
     }
   }
 }
@@ -2767,7 +2768,7 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
     HIP_TRY(hipMemcpyAsync(dsts[k], tmp, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, m.st));
   }
   std::vector<int> nreb(m.S), lc(m.S);
-  int ints[4];
+  int ints[6];
   HIP_TRY(hipMemcpyAsync(nreb.data(), m.nreb, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
   HIP_TRY(hipMemcpyAsync(lc.data(), m.log_count, m.S * sizeof(int), hipMemcpyDeviceToHost, m.st));
   HIP_TRY(hipMemcpyAsync(ints, m.ints, sizeof ints, hipMemcpyDeviceToHost, m.st));
@@ -2803,7 +2804,8 @@ int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host
     long long tp[6];
     HIP_TRY(hipMemcpy(tp, m.tphase, sizeof tp, hipMemcpyDeviceToHost));
     for (int k = 1; k < 6; ++k) info[4 + k] = tp[k] - tp[0];
-    info[10] = info[11] = 0;
+    info[10] = ints[5];  // rebuilds in which some cell outgrew its bucket
+    info[11] = m.bcap;
     int dev = 0, khz = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));

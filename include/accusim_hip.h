@@ -175,8 +175,9 @@ int accu_bmd_sync(accu_bmd* h);
  * per record (Simulation.record, simulation.py:76-93); stats [nsys][4] (rebuilds, records);
  * info [16] = max neighbours, capacity, region capacity, region overflow, list entries,
  * then the last rebuild's phase ends in device wall-clock ticks after its start
- * ([5] binning, [6] block sums, [7] scan, [8] grouping by cell, [9] in-cell order and
- * gather), [12] the wall-clock rate in kHz */
+ * ([5] binning, [6] scan, [7] grouping by cell (only when a cell outgrew its arrival
+ * bucket), [8] = [7], [9] in-cell order and gather), [10] rebuilds in which a cell outgrew
+ * its bucket, [11] the bucket capacity per cell, [12] the wall-clock rate in kHz */
 int accu_bmd_get(accu_bmd* h, double* pos_host, double* vel_host, double* F_host, double* log_host, int log_max,
                  long long* stats_host, long long* info_host);
 /* workgroup size of the per-brick kernels (multiple of 64 in [64, 512]; default 256) */

@@ -150,3 +150,47 @@ def test_list_reuse_conserves_energy_like_rebuilding_every_step(precision):
         eng.close()
     assert drifts[0] < 1e-4, drifts
     assert abs(ends[0] - ends[1]) < 5e-5, ends
+
+
+def test_rebuild_groups_by_slot_when_a_cell_outgrows_its_bucket():
+    """The rebuild files atoms into per-cell buckets sized at set-up (twice the fullest cell,
+    plus 8, in multiples of 8); a rebuild in which a cell outgrows its bucket groups the
+    atoms by their arrival slots instead. Two atoms per cell at start (bucket of 16), then
+    the 3 x 3 cells around the centre of one layer contract in x and y to a third of their
+    size by t = 1 (dt 0.005, 210 steps): 18 atoms in one cell at the last rebuilds. Weak LJ (eps 1e-6) keeps the contraction
+    ballistic. The trajectory and the rebuild count equal the oracle's either way."""
+    nc, h = 10, 2.83
+    L = nc * h
+    g = (np.arange(nc) + 0.5) * h
+    cen = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = np.concatenate([cen - [0.25 * h, 0, 0], cen + [0.25 * h, 0, 0]])
+    rng = np.random.default_rng(3)
+    pos = pos + rng.normal(0, 1e-3, pos.shape)
+    c = np.full(3, g[5])
+    block = (np.abs(pos[:, 0] - c[0]) < 1.5 * h) & (np.abs(pos[:, 1] - c[1]) < 1.5 * h) & \
+            (np.abs(pos[:, 2] - c[2]) < 0.5 * h)
+    assert block.sum() == 18
+    T = 1.0
+    vel = np.zeros_like(pos)
+    vel[block, :2] = -(pos[block, :2] - c[:2]) * (2.0 / 3.0) / T
+    s = A.System(3, box=L, periodic=True)
+    s.add_species("A", 1.0)
+    s.add("A", pos)
+    s.vel = vel
+    so = s.copy()
+    lj = {"A": (1e-6, 1.0)}
+    eng = batch.BrickEngine([s], [[dp.LennardJones(lj, cutoff=2.5, skin=0.3)]], dp.VelocityVerlet(), 0.005,
+                            bricks=[(1, 1, 1)], slack=4.0)
+    eng.run(210)
+    eng.fetch()
+    sim = A.Simulation(so, [A.LennardJones(lj, cutoff=2.5, skin=0.3)], integrator=A.VelocityVerlet(), dt=0.005)
+    tr = sim.run(steps=210, positions=False)
+    print("rebuilds", int(eng.rebuilds[0]), "overflowing", eng.info["bucket_overflows"], flush=True)
+    assert eng.info["bucket_cap"] == 16
+    assert eng.info["bucket_overflows"] >= 1
+    assert int(eng.rebuilds[0]) > eng.info["bucket_overflows"]  # the earlier rebuilds used the buckets
+    assert int(eng.rebuilds[0]) == sim.forces[0].rebuilds
+    d = s.pos - so.pos
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) <= 1e-10 * L
+    assert np.allclose(eng.logs[0][:, 0], tr.potential, rtol=1e-9, atol=1e-15)
