@@ -562,43 +562,82 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   //    the other systems keep their counts. Atoms are still nearly in cell order, so a
   //    wave's lanes share a few cells: one atomic per distinct cell per wave (the lanes
   //    of a cell take consecutive slots in lane order).
+  //    A wave takes kBinU chunks of 64 atoms at once and issues the atomics of all their
+  //    distinct cells before it waits for any return value.
   {
+    constexpr int kBinU = 4;
     const int lane = tid & 63;
-    for (int a0 = gtid - lane; a0 < p.N; a0 += gsz) {
-      const int a = a0 + lane;
-      int gid = -1;
-      if (a < p.N) {
-        const int si = p.sysid[a];
-        if (p.flags[si]) {
-          const BSys& s = p.sys[si];
-          const double4 x = (rb_par(p, si) ? p.posB : p.posA)[a];
-          const double xs[3] = {x.x, x.y, x.z};
-          int c[3];
+    for (int a00 = gtid - lane; a00 < p.N; a00 += kBinU * gsz) {
+      int gid[kBinU], ret[kBinU];
+      // the cell of atom a of system si (flagged for a rebuild); gid -1 otherwise
+      // (every table value is loaded before the first is used: one round trip, then the
+      // position, then the cell map)
+      auto bin_cell = [&](int a, int si) -> int {
+        const BSys& s = p.sys[si];
+        const int fl = p.flags[si], pr = rb_par(p, si), spr = p.spar[si];
+        const double ci[3] = {s.cinv[0], s.cinv[1], s.cinv[2]};
+        const int nc[3] = {s.nc[0], s.nc[1], s.nc[2]}, moff = s.cellmap_off;
+        if (!fl) return -1;
+        const double4 x = (pr ? p.posB : p.posA)[a];
+        const double xs[3] = {x.x, x.y, x.z};
+        int c[3];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const double t = xs[d] * s.cinv[d];
-            const int m = s.nc[d] - 1;
-            c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : (int)t);
-          }
-          gid = p.cellmap[s.cellmap_off + (c[0] * s.nc[1] + c[1]) * s.nc[2] + c[2]];
-          p.st.cell[p.spar[si]][a] = gid;
+        for (int d = 0; d < 3; ++d) {
+          const double t = xs[d] * ci[d];
+          const int m = nc[d] - 1;
+          const int tr = (int)fmin(fmax(t, 0.0), (double)m);  // t in [0, m): its truncation
+          c[d] = !(t >= 0.0) ? 0 : (t >= (double)m ? m : tr);
+        }
+        const int g = p.cellmap[moff + (c[0] * nc[1] + c[1]) * nc[2] + c[2]];
+        p.st.cell[spr][a] = g;
+        return g;
+      };
+#pragma unroll
+      for (int u = 0; u < kBinU; ++u) {
+        const int a = a00 + u * gsz + lane;
+        gid[u] = -1;
+        ret[u] = 0;
+        if (a00 + u * gsz >= p.N) continue;  // the whole chunk is past the end (uniform)
+        const int si_l = a < p.N ? p.sysid[a] : -1;
+        // a chunk inside one system (the common case) reads that system's tables through
+        // scalar loads; a chunk across systems per lane
+        const int si0 = __builtin_amdgcn_readfirstlane(si_l);
+        if (__all(a >= p.N || si_l == si0)) {
+          if (a < p.N) gid[u] = bin_cell(a, si0);
+        } else if (a < p.N) {
+          gid[u] = bin_cell(a, si_l);
         }
       }
-      unsigned long long todo = __ballot(gid >= 0);
-      while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int g = __shfl(gid, leader, 64);
-        const unsigned long long same = __ballot(gid == g);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&p.count[g], __popcll(same));
-        base = __shfl(base, leader, 64);
-        if (gid == g) {
-          const int sl = base + __popcll(same & ((1ull << lane) - 1ull));
-          p.slot[a] = sl;
-          if (sl < p.bcap) p.bucket[(size_t)g * p.bcap + sl] = a;
-          else if (p.bcap) atomicOr(&p.ints[4], 1);
+      // the leader lane of each distinct cell reserves that cell's slots (returns unused yet)
+#pragma unroll
+      for (int u = 0; u < kBinU; ++u) {
+        unsigned long long todo = __ballot(gid[u] >= 0);
+        while (todo) {
+          const int leader = __ffsll((long long)todo) - 1;
+          const int g = __shfl(gid[u], leader, 64);
+          const unsigned long long same = __ballot(gid[u] == g);
+          if (lane == leader) ret[u] = atomicAdd(&p.count[g], __popcll(same));
+          todo &= ~same;
         }
-        todo &= ~same;
+      }
+      // the same walk again: each lane's slot from its leader's reservation
+#pragma unroll
+      for (int u = 0; u < kBinU; ++u) {
+        const int a = a00 + u * gsz + lane;
+        unsigned long long todo = __ballot(gid[u] >= 0);
+        while (todo) {
+          const int leader = __ffsll((long long)todo) - 1;
+          const int g = __shfl(gid[u], leader, 64);
+          const unsigned long long same = __ballot(gid[u] == g);
+          const int base = __shfl(ret[u], leader, 64);
+          if (gid[u] == g) {
+            const int sl = base + __popcll(same & ((1ull << lane) - 1ull));
+            p.slot[a] = sl;
+            if (sl < p.bcap) p.bucket[(size_t)g * p.bcap + sl] = a;
+            else if (p.bcap) atomicOr(&p.ints[4], 1);
+          }
+          todo &= ~same;
+        }
       }
     }
   }
@@ -685,6 +724,8 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
   //    the whole wave takes the two in turn (shuffle rank up to 64 atoms, a rank against
   //    the cell's atoms in memory beyond that). Control flow stays wave-uniform.
   {
+    __shared__ double rz[kRB];
+    __shared__ int rv[kRB];
     const int lane = tid & 63, half = lane >> 5, hl = lane & 31;
     const int gw = gtid >> 6, nw = gsz >> 6;
     auto gather = [&](int c, int t, int v, int si) {
@@ -699,25 +740,56 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       p.st.cell[sc ^ 1][t] = c;
       if (p.islot) p.islot[u] = t;
     };
+    // a half-wave's cell of pair cp: whether it is rebuilt, its start and length, and (for a
+    // cell of at most 32 atoms) this lane's atom and key; the next pair's are loaded while
+    // the current one is ranked and gathered
+    struct Pair {
+      int si, a, len, v;
+      double zv;
+    };
+    auto load_pair = [&](int cp) {
+      Pair q{0, 0, 0, 0x7fffffff, INFINITY};
+      const int cm = cp + half;
+      if (cm < p.ncells) {
+        q.si = p.cellsys[cm];
+        const int a0 = p.start[cm], a1 = p.start[cm + 1];
+        if (p.flags[q.si]) {
+          q.a = a0;
+          q.len = a1 - a0;
+          if (hl < q.len && q.len <= 32) {
+            q.v = atoms_of(cm)[hl];
+            q.zv = zkey(p, q.v, q.si);
+          }
+        }
+      }
+      return q;
+    };
+    Pair nxt = load_pair(2 * gw);
     for (int cp = 2 * gw; cp < p.ncells; cp += 2 * nw) {
       const int cm = cp + half;  // this half-wave's cell
-      int len = 0, a = 0;
-      const bool on = cm < p.ncells && p.flags[p.cellsys[cm]];
-      if (on) {
-        a = p.start[cm];
-        len = p.start[cm + 1] - a;
-      }
+      const Pair cur = nxt;
+      if (cp + 2 * nw < p.ncells) nxt = load_pair(cp + 2 * nw);
+      const int len = cur.len, a = cur.a;
+      const bool on = len > 0;
       if (!__any(len > 32)) {
         const bool mine = on && hl < len;
-        const int v = mine ? atoms_of(cm)[hl] : 0x7fffffff;
-        const double zv = mine ? zkey(p, v, p.cellsys[cm]) : INFINITY;
+        const int v = cur.v;
+        const double zv = cur.zv;
+        // the half-wave's keys through the LDS (lane slots; the unused ones hold keys that
+        // rank after every atom), read back as broadcasts up to the longer cell of the pair
+        rz[tid] = zv;
+        rv[tid] = v;
+        __builtin_amdgcn_wave_barrier();
+        const int lmax = max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 32));
+        const int hb = tid & ~31;
         int r = 0;
-        for (int x = 0; x < 32; ++x) {
-          const double zx = __shfl(zv, (half << 5) + x, 64);
-          const int vx = __shfl(v, (half << 5) + x, 64);
+        for (int x = 0; x < lmax; ++x) {
+          const double zx = rz[hb + x];
+          const int vx = rv[hb + x];
           r += (zx < zv || (zx == zv && vx < v)) ? 1 : 0;
         }
-        if (mine) gather(cm, a + r, v, p.cellsys[cm]);
+        __builtin_amdgcn_wave_barrier();
+        if (mine) gather(cm, a + r, v, cur.si);
       } else {
         for (int k = 0; k < 2; ++k) {
           const int c = cp + k;
