@@ -202,15 +202,42 @@ __device__ void grid_sum(double (&v)[NV], double* partials, unsigned* counter, d
   }
 }
 
-// Per-call reduction workspace owned by the library (one per entry-point family).
+// Scratch of the handle-free entry points (reduction partials and counter, the Ewald
+// reciprocal buffers), one per (device, stream): calls on different streams or devices never
+// share it, and calls on one stream use it in stream order. Calls on one stream from several
+// host threads at once must be serialised by the caller.
 struct Workspace {
+  int dev = 0;
+  hipStream_t st = nullptr;
   double* partials = nullptr;
   size_t cap = 0;  // doubles
   unsigned* counter = nullptr;
   double* out = nullptr;  // 16 doubles
+  // Ewald reciprocal sum: partial structure factors, weighted S(k), weights, columns
+  double2* ew_part = nullptr;
+  double2* ew_A = nullptr;
+  double* ew_kw = nullptr;
+  int* ew_col = nullptr;
+  long long* ew_off = nullptr;
+  size_t ew_part_cap = 0, ew_A_cap = 0, ew_kw_cap = 0, ew_col_cap = 0, ew_off_cap = 0;  // elements
   int ensure(size_t ndoubles);
+  // device buffer p grown to at least n elements of T (its contents are not kept); the
+  // old one is freed after the stream's earlier work
+  template <typename T>
+  int grow(T*& p, size_t& cap_n, size_t n) {
+    if (n <= cap_n && p) return 0;
+    if (p) {
+      if (hipStreamSynchronize(st) != hipSuccess || hipFree(p) != hipSuccess) return grow_error();
+      p = nullptr;
+    }
+    if (hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T)) != hipSuccess) return grow_error();
+    cap_n = n;
+    return 0;
+  }
+  int grow_error();
 };
 
-Workspace& default_workspace();
+// The workspace of calls on `stream` on the current device (created at its first use).
+Workspace* workspace_for(hipStream_t stream);
 
 }  // namespace accu

@@ -3,6 +3,10 @@
 #include "../../include/accusim_hip.h"
 
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <utility>
 
 namespace accu {
 
@@ -15,6 +19,11 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+int Workspace::grow_error() {
+  set_error("workspace allocation failed: %s", hipGetErrorString(hipGetLastError()));
+  return ACCU_ERUNTIME;
+}
+
 int Workspace::ensure(size_t ndoubles) {
   if (!counter) {
     HIP_TRY(hipMalloc(&counter, 64));
@@ -23,17 +32,28 @@ int Workspace::ensure(size_t ndoubles) {
     HIP_TRY(hipMalloc(&out, 16 * sizeof(double)));
   }
   if (ndoubles > cap) {
-    if (partials) HIP_TRY(hipFree(partials));
     size_t c = ndoubles < 4096 ? 4096 : ndoubles;
-    HIP_TRY(hipMalloc(&partials, c * sizeof(double)));
-    cap = c;
+    if (grow(partials, cap, c)) return ACCU_ERUNTIME;
   }
   return ACCU_OK;
 }
 
-Workspace& default_workspace() {
-  static Workspace ws;
-  return ws;
+Workspace* workspace_for(hipStream_t stream) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> all;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("no current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  std::unique_ptr<Workspace>& w = all[std::make_pair(dev, stream)];
+  if (!w) {
+    w.reset(new Workspace());
+    w->dev = dev;
+    w->st = stream;
+  }
+  return w.get();
 }
 
 }  // namespace accu

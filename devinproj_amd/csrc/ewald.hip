@@ -189,19 +189,6 @@ __global__ void k_scale1(double* v, double f) {
   if (threadIdx.x == 0 && blockIdx.x == 0) v[0] *= f;
 }
 
-struct EwaldScratch {
-  double2* part = nullptr;
-  size_t part_cap = 0;
-  double2* A = nullptr;
-  double* kw = nullptr;
-  size_t k_cap = 0;
-  int* col = nullptr;
-  long long* off = nullptr;
-  size_t col_cap = 0;
-};
-
-static EwaldScratch g_ew;
-
 }  // namespace accu
 
 using namespace accu;
@@ -227,34 +214,25 @@ int accu_ewald_recip(const double* pos, const double* q, int n, const double* bo
     maxlen = len > maxlen ? len : maxlen;
   }
   ARG_CHECK(maxlen <= kMaxCol, "k column of %d values exceeds %d", maxlen, kMaxCol);
-  EwaldScratch& s = g_ew;
-  if ((size_t)ncol > s.col_cap) {
-    if (s.col) HIP_TRY(hipFree(s.col));
-    if (s.off) HIP_TRY(hipFree(s.off));
-    HIP_TRY(hipMalloc(&s.col, (size_t)ncol * 4 * sizeof(int)));
-    HIP_TRY(hipMalloc(&s.off, (size_t)ncol * sizeof(long long)));
-    s.col_cap = ncol;
-  }
-  if ((size_t)K > s.k_cap) {
-    if (s.A) HIP_TRY(hipFree(s.A));
-    if (s.kw) HIP_TRY(hipFree(s.kw));
-    HIP_TRY(hipMalloc(&s.A, (size_t)K * sizeof(double2)));
-    HIP_TRY(hipMalloc(&s.kw, (size_t)K * sizeof(double)));
-    s.k_cap = K;
-  }
+  Workspace* ws = workspace_for(st);
+  if (!ws) return ACCU_ERUNTIME;
   const int npb = (n + kET * kEM - 1) / (kET * kEM);
-  if ((size_t)npb * K > s.part_cap) {
-    if (s.part) HIP_TRY(hipFree(s.part));
-    HIP_TRY(hipMalloc(&s.part, (size_t)npb * K * sizeof(double2)));
-    s.part_cap = (size_t)npb * K;
-  }
+  if (ws->grow(ws->ew_col, ws->ew_col_cap, (size_t)ncol * 4) || ws->grow(ws->ew_off, ws->ew_off_cap, (size_t)ncol) ||
+      ws->grow(ws->ew_A, ws->ew_A_cap, (size_t)K) || ws->grow(ws->ew_kw, ws->ew_kw_cap, (size_t)K) ||
+      ws->grow(ws->ew_part, ws->ew_part_cap, (size_t)npb * K))
+    return ACCU_ERUNTIME;
+  struct {
+    int* col;
+    long long* off;
+    double2 *A, *part;
+    double* kw;
+  } s{ws->ew_col, ws->ew_off, ws->ew_A, ws->ew_part, ws->ew_kw};
   HIP_TRY(hipMemcpyAsync(s.col, cols_host, (size_t)ncol * 4 * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(s.off, off.data(), (size_t)ncol * sizeof(long long), hipMemcpyHostToDevice, st));
   EwCols ec{s.col, s.off, ncol, {2 * M_PI / box[0], 2 * M_PI / box[1], 2 * M_PI / box[2]}};
   const double V = box[0] * box[1] * box[2];
-  Workspace& ws = default_workspace();
   const int nbk = (int)((K + kET - 1) / kET);
-  int rc = ws.ensure((size_t)nbk * 2 + 16);
+  int rc = ws->ensure((size_t)nbk * 2 + 16);
   if (rc) return rc;
   hipLaunchKernelGGL(k_kweights, dim3((ncol + 255) / 256), dim3(256), 0, st, ec, 1.0 / (4 * alpha * alpha), s.kw);
   // enough column chunks to give ~4 blocks per CU
@@ -262,8 +240,8 @@ int accu_ewald_recip(const double* pos, const double* q, int n, const double* bo
   cpb = cpb < 1 ? 1 : (cpb > 64 ? 64 : cpb);
   const int nchunk = (ncol + cpb - 1) / cpb;
   hipLaunchKernelGGL(k_sk_partial, dim3(nchunk, npb), dim3(kET), 0, st, pos, q, n, ec, cpb, K, s.part);
-  double* Ud = U ? U : ws.out;
-  hipLaunchKernelGGL(k_sk_reduce, dim3(nbk), dim3(kET), 0, st, s.part, npb, K, s.kw, s.A, ws.partials, ws.counter,
+  double* Ud = U ? U : ws->out;
+  hipLaunchKernelGGL(k_sk_reduce, dim3(nbk), dim3(kET), 0, st, s.part, npb, K, s.kw, s.A, ws->partials, ws->counter,
                      Ud);
   // U_rec = 2 * (2 pi / V) * sum_half w |S|^2, times the Coulomb constant
   if (U) hipLaunchKernelGGL(k_scale1, dim3(1), dim3(64), 0, st, U, 4.0 * M_PI / V * k);

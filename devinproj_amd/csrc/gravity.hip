@@ -230,11 +230,12 @@ int accu_gravity_direct(const double* pos, const double* mass, int n, const long
   if (!targets) ntgt = n;
   if (n <= 0 || ntgt <= 0) return ACCU_OK;
   const GravPlan g = plan_gravity(n, ntgt, targets != nullptr, U != nullptr, precision);
-  Workspace& ws = default_workspace();
-  rc = ws.ensure(g.need);
+  Workspace* ws = workspace_for((hipStream_t)stream);
+  if (!ws) return ACCU_ERUNTIME;
+  rc = ws->ensure(g.need);
   if (rc) return rc;
-  return launch_gravity(g, pos, mass, n, targets, ntgt, G, eps2, box, periodic, precision, origin, F, U, ws.partials,
-                        ws.counter, (hipStream_t)stream);
+  return launch_gravity(g, pos, mass, n, targets, ntgt, G, eps2, box, periodic, precision, origin, F, U, ws->partials,
+                        ws->counter, (hipStream_t)stream);
 }
 
 }  // extern "C"

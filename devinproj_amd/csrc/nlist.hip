@@ -561,8 +561,9 @@ int accu_max_disp2(const double* pos, const double* ref, int n, const double* bo
   hipStream_t st = (hipStream_t)stream;
   *out_host = 0.0;
   if (n <= 0) return ACCU_OK;
-  Workspace& ws = default_workspace();
-  int rc = ws.ensure(16);
+  Workspace* ws = workspace_for(st);
+  if (!ws) return ACCU_ERUNTIME;
+  int rc = ws->ensure(16);
   if (rc) return rc;
   Box b{};
   b.periodic = periodic;
@@ -570,7 +571,7 @@ int accu_max_disp2(const double* pos, const double* ref, int n, const double* bo
     b.L[d] = box ? box[d] : 1.0;
     b.inv[d] = 1.0 / b.L[d];
   }
-  unsigned long long* o = reinterpret_cast<unsigned long long*>(ws.out);
+  unsigned long long* o = reinterpret_cast<unsigned long long*>(ws->out);
   HIP_TRY(hipMemsetAsync(o, 0, sizeof(unsigned long long), st));
   const int T = 256;
   hipLaunchKernelGGL(k_maxdisp2, dim3((n + T - 1) / T), dim3(T), 0, st, pos, 3, ref, 3, n, b, o);

@@ -242,21 +242,22 @@ int accu_lj_forces(accu_nlist* hh, const double* pos, const int* sp, int n, int 
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (n <= 0) return ACCU_OK;
-  Workspace& ws = default_workspace();
+  Workspace* ws = workspace_for(st);
+  if (!ws) return ACCU_ERUNTIME;
   const int nb = (n + kT - 1) / kT, nbl = (int)((4LL * n + kT - 1) / kT);  // dense / list (a quad per atom)
-  if ((rc = ws.ensure((size_t)nbl * 2))) return rc;
+  if ((rc = ws->ensure((size_t)nbl * 2))) return rc;
   Box b = make_box(box, periodic);
   if (hh) {
     NList& h = *reinterpret_cast<NList*>(hh);
     ARG_CHECK(h.n == n, "neighbour list was built for %d atoms, got %d", h.n, n);
     if (precision == ACCU_FP64)
       hipLaunchKernelGGL(k_lj_list<double>, dim3(nbl), dim3(kT), 0, st, prm, pos, 3, sp, n, h.sorted, h.nn, h.nbr,
-                         h.stride, b, F, ws.partials, ws.counter, U);
+                         h.stride, b, F, ws->partials, ws->counter, U);
     else
       hipLaunchKernelGGL(k_lj_list<float>, dim3(nbl), dim3(kT), 0, st, prm, pos, 3, sp, n, h.sorted, h.nn, h.nbr,
-                         h.stride, b, F, ws.partials, ws.counter, U);
+                         h.stride, b, F, ws->partials, ws->counter, U);
   } else {
-    hipLaunchKernelGGL(k_lj_dense, dim3(nb), dim3(kT), 0, st, prm, pos, sp, n, b, F, ws.partials, ws.counter, U);
+    hipLaunchKernelGGL(k_lj_dense, dim3(nb), dim3(kT), 0, st, prm, pos, sp, n, b, F, ws->partials, ws->counter, U);
   }
   LAUNCH_CHECK();
   return ACCU_OK;
@@ -283,9 +284,10 @@ int accu_coulomb_forces(accu_nlist* hh, const double* pos, const double* q, int 
   p.shifted = shifted;
   p.nex = nex;
   p.ex = ex_keys;
-  Workspace& ws = default_workspace();
+  Workspace* ws = workspace_for(st);
+  if (!ws) return ACCU_ERUNTIME;
   const int nb = (n + kT - 1) / kT, nbl = (int)((4LL * n + kT - 1) / kT);  // dense / list (a quad per atom)
-  int rc = ws.ensure((size_t)nbl * 2);
+  int rc = ws->ensure((size_t)nbl * 2);
   if (rc) return rc;
   Box b = make_box(box, periodic);
   if (hh) {
@@ -293,12 +295,12 @@ int accu_coulomb_forces(accu_nlist* hh, const double* pos, const double* q, int 
     ARG_CHECK(h.n == n, "neighbour list was built for %d atoms, got %d", h.n, n);
     if (precision == ACCU_FP64)
       hipLaunchKernelGGL(k_coul_list<double>, dim3(nbl), dim3(kT), 0, st, p, pos, 3, q, n, h.sorted, h.nn, h.nbr,
-                         h.stride, b, F, ws.partials, ws.counter, U);
+                         h.stride, b, F, ws->partials, ws->counter, U);
     else
       hipLaunchKernelGGL(k_coul_list<float>, dim3(nbl), dim3(kT), 0, st, p, pos, 3, q, n, h.sorted, h.nn, h.nbr,
-                         h.stride, b, F, ws.partials, ws.counter, U);
+                         h.stride, b, F, ws->partials, ws->counter, U);
   } else {
-    hipLaunchKernelGGL(k_coul_dense, dim3(nb), dim3(kT), 0, st, p, pos, q, n, b, F, ws.partials, ws.counter, U);
+    hipLaunchKernelGGL(k_coul_dense, dim3(nb), dim3(kT), 0, st, p, pos, q, n, b, F, ws->partials, ws->counter, U);
   }
   LAUNCH_CHECK();
   return ACCU_OK;

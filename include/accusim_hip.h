@@ -14,7 +14,12 @@
  *     and ACCU_EOVERFLOW (-3) to RuntimeError; accu_last_error() has the message.
  *   - Work is enqueued on `stream` (a hipStream_t); outputs are caller-owned.
  *   - U arguments are device float64 scalars and may be NULL (no energy).
- *   - Handles are not thread-safe; the library keeps per-process scratch memory.
+ *   - Handles are not thread-safe (one host thread at a time per handle). The entry points
+ *     without a handle (pair forces, gravity, Ewald, accu_max_disp2) keep their scratch
+ *     (reduction partials, grid-sum counter, Ewald buffers) per (device, stream): calls
+ *     on different streams never share it, calls on one stream use it in stream order,
+ *     and calls on one stream from several host threads at once must be serialised by
+ *     the caller. The first call on a stream allocates that stream's scratch.
  */
 #ifndef ACCUSIM_HIP_H
 #define ACCUSIM_HIP_H
