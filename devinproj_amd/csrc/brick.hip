@@ -494,17 +494,20 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
         int lo = rs[r0];
         const int qe = rs[r0 + 3];
         while (lo + 8 < qe && ld_slot<PK3>(lds32, lo + 8).z - xi.z <= -rcw) lo += 8;
-        // four candidates per step: their LDS reads are in flight together (the slots
-        // past the run's end re-read its last atom and are rejected by q < qe)
+        // the atom itself is a candidate only in its own column
+        const int skip = (ox == 1 && oy == 1) ? self : -1;
+        // four candidates per step: their LDS reads are in flight together, at one base
+        // address plus fixed offsets (the slots past the run's end, at most 3 of them and
+        // inside the LDS padding, are read and rejected by q < qe)
         for (int q = lo; q < qe; q += 4) {
           float4 xj[4];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) xj[k] = ld_slot<PK3>(lds32, min(q + k, qe - 1));
+          for (int k = 0; k < 4; ++k) xj[k] = ld_slot<PK3>(lds32, q + k);
           if (xj[0].z - xi.z >= rcw) break;  // this and every later candidate is above the window
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
             const float dx = xj[k].x - xi.x, dy = xj[k].y - xi.y, dz = xj[k].z - xi.z;
-            if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != self) {
+            if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != skip) {
               w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
               w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
               w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
@@ -1911,8 +1914,11 @@ struct BMD {
   // the list launch stages 12-byte slots when 16-byte ones would keep fewer than 4 bricks
   // of 512 threads on a CU (160 KiB of LDS); 16-byte slots read in one instruction
   bool pk3_build() const { return (size_t)cap_build * sizeof(float4) > 36 * 1024; }
+  // (4 slots past the capacity: the list build reads up to 3 slots beyond a column's last
+  // one and discards them, so its candidate addresses need no clamp)
   size_t lds_build() const {
-    return pk3_build() ? ((size_t)cap_build * sizeof(Pk3) + 15) / 16 * 16 : (size_t)cap_build * sizeof(float4);
+    return pk3_build() ? ((size_t)(cap_build + 4) * sizeof(Pk3) + 15) / 16 * 16
+                       : (size_t)(cap_build + 4) * sizeof(float4);
   }
 
   RB rebuild_args(int bin_only, int flip_n, int rec = 0) const {
