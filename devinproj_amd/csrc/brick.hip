@@ -478,7 +478,7 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
     // 8 entries per 16-byte group: each hit shifts the 128-bit register down by 16 bits
     // and enters at the top, so the first entry of a group ends in the low half of w0
-    int cnt = 0;
+    int cnt = 0, full = 7;  // full: the count at which the current group is complete
     unsigned w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     for (int ox = 0; ox < 3; ++ox) {
       const int rx = lx + ox;
@@ -512,8 +512,10 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
               w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
               w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
               w3 = __builtin_amdgcn_alignbit((unsigned)(q + k), w3, 16);
-              if ((cnt & 7) == 7 && (cnt >> 3) < p.ngroups)
-                p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+              if (cnt == full) {  // the eighth entry of a group: store the group
+                if ((cnt >> 3) < p.ngroups) p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+                full += 8;
+              }
               ++cnt;
             }
           }
