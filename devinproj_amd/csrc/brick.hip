@@ -1883,6 +1883,14 @@ struct BMD {
   long long rec_enq = 0;  // records enqueued so far (the set-up record is record 0)
   int cap_fused = 0, cap_build = 0;  // region capacity (atoms) for the LDS
   int threads = 256;                 // workgroup size of the per-brick kernels (64..512)
+  double home_mean = 0.0;            // home atoms per brick at set-up (mean over bricks)
+  // The list launch takes one thread per home atom: its workgroup is the multiple of 64 just
+  // above 1.1x the mean brick (a larger brick takes a second pass), at most `threads`, so
+  // that small bricks do not leave most lanes idle.
+  int build_threads() const {
+    const int t = ((int)(1.1 * home_mean) + 63) / 64 * 64;
+    return std::max(64, std::min(threads, t));
+  }
   bool ready = false;
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -1989,8 +1997,8 @@ struct BMD {
   void launch_rebuild(int flip_n, int rec = 0) {
     const RB p = rebuild_args(0, flip_n, rec);
     hipLaunchKernelGGL(k_b_rebuild, dim3(grid_rb), dim3(kRB), 0, st, p);
-    hipLaunchKernelGGL(pk3_build() ? k_b_build<true> : k_b_build<false>, dim3(nbricks), dim3(threads), lds_build(),
-                       st, p);
+    hipLaunchKernelGGL(pk3_build() ? k_b_build<true> : k_b_build<false>, dim3(nbricks), dim3(build_threads()),
+                       lds_build(), st, p);
   }
 
   template <typename P, bool ONE, bool EN>
@@ -2496,6 +2504,11 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   }
   std::vector<BBrick> hb(m.nbricks);
   HIP_TRY(hipMemcpy(hb.data(), m.d_bricks, m.nbricks * sizeof(BBrick), hipMemcpyDeviceToHost));
+  {
+    double home = 0.0;
+    for (auto& b : hb) home += hstart[b.cell0 + b.nhome] - hstart[b.cell0];
+    m.home_mean = m.nbricks ? home / m.nbricks : 0.0;
+  }
   int regmax = 0;
   {
     std::vector<BRegion> hr;
