@@ -387,6 +387,14 @@ __global__ void __launch_bounds__(kThreads) k_moments(int nn, const int* __restr
 }
 
 // ---- the walk (tree.py:54-78) -------------------------------------------------------
+//
+// Each lane walks for one target (particles in tree order, so a wave's 64 targets are
+// neighbours). The wave steps through the tree together: the node is the same for every
+// lane, so its data loads are shared, and a lane that accepted a node (or finished a
+// leaf) skips that node's subtree, [node, next[node]), while the others descend. The wave
+// moves to the first child when some lane still needs it, else past the subtree. Every
+// lane therefore meets exactly the nodes and leaves of its own stackless walk, in the same
+// pre-order, and sums the same terms in the same order as a walk of its own would.
 
 template <bool QUAD>
 __global__ void __launch_bounds__(kThreads) k_bh_walk(int n, int nnodes, const double4* __restrict__ com,
@@ -399,12 +407,16 @@ __global__ void __launch_bounds__(kThreads) k_bh_walk(int n, int nnodes, const d
   __shared__ double red[16];
   const int k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   double U[1] = {0.0};
-  if (k < n) {
-    const double4 p = ppos[k];
-    double ax = 0, ay = 0, az = 0, phi = 0;
-    int node = 0;
-    while (node < nnodes) {
-      if (leaf[node]) {
+  const bool on = k < n;
+  const double4 p = on ? ppos[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+  double ax = 0, ay = 0, az = 0, phi = 0;
+  int skip = on ? 0 : nnodes;  // this lane sits out the nodes below skip (an accepted subtree)
+  int node = 0;                // the wave's node (uniform)
+  while (node < nnodes) {
+    const bool act = node >= skip;
+    const int nx = next[node];
+    if (leaf[node]) {
+      if (act) {
         const int a = lstart[node], b = a + lcount[node];
         for (int s = a; s < b; ++s) {
           if (s == k) continue;
@@ -418,9 +430,12 @@ __global__ void __launch_bounds__(kThreads) k_bh_walk(int n, int nnodes, const d
           az = fma(mi3, dz, az);
           phi = fma(-q.w, inv, phi);
         }
-        node = next[node];
-        continue;
       }
+      node = nx;
+      continue;
+    }
+    bool descend = false;
+    if (act) {
       const double4 c = com[node];
       const double4 cb = cube[node];
       const double dx = c.x - p.x, dy = c.y - p.y, dz = c.z - p.z;
@@ -449,11 +464,14 @@ __global__ void __launch_bounds__(kThreads) k_bh_walk(int n, int nnodes, const d
           az += QRz * inv5 - 2.5 * RQR * Rz * inv7;
           phi -= 0.5 * RQR * inv5;
         }
-        node = next[node];
+        skip = nx;
       } else {
-        node = node + 1;  // first child
+        descend = true;
       }
     }
+    node = __any(descend) ? node + 1 : nx;  // first child, or past the subtree
+  }
+  if (on) {
     const int i = perm[k];
     F[3 * (size_t)i] = G * p.w * ax;
     F[3 * (size_t)i + 1] = G * p.w * ay;
