@@ -746,14 +746,22 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       if (p.islot) p.islot[u] = t;
     };
     // a half-wave's cell of pair cp: whether it is rebuilt, its start and length, and (for a
-    // cell of at most 32 atoms) this lane's atom and key; the next pair's are loaded while
-    // the current one is ranked and gathered
+    // cell of at most 32 atoms) this lane's atom, its position (z is the key), velocity and
+    // user index; the next pair's are loaded while the current one is ranked and gathered,
+    // so the gather's loads are in flight during the ranking
     struct Pair {
-      int si, a, len, v;
-      double zv;
+      int si, a, len, v, u;
+      double4 x, vv;
     };
     auto load_pair = [&](int cp) {
-      Pair q{0, 0, 0, 0x7fffffff, INFINITY};
+      Pair q;
+      q.si = 0;
+      q.a = 0;
+      q.len = 0;
+      q.v = 0x7fffffff;
+      q.u = 0;
+      q.x = make_double4(0.0, 0.0, INFINITY, 0.0);
+      q.vv = make_double4(0.0, 0.0, 0.0, 0.0);
       const int cm = cp + half;
       if (cm < p.ncells) {
         q.si = p.cellsys[cm];
@@ -763,11 +771,25 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
           q.len = a1 - a0;
           if (hl < q.len && q.len <= 32) {
             q.v = atoms_of(cm)[hl];
-            q.zv = zkey(p, q.v, q.si);
+            const int pc = rb_par(p, q.si), sc = p.spar[q.si];
+            q.x = (pc ? p.posB : p.posA)[q.v];
+            q.vv = p.st.vel[sc][q.v];
+            q.u = p.st.perm[sc][q.v];
           }
         }
       }
       return q;
+    };
+    // the gather of a preloaded atom (the small-cell path)
+    auto gather_pre = [&](int c, int t, const Pair& q) {
+      const int pc = rb_par(p, q.si), sc = p.spar[q.si];
+      (pc ? p.posA : p.posB)[t] = q.x;
+      if (p.st.posf[0]) p.st.posf[pc ^ 1][t] = fix_pos(p.sys[q.si], q.x, p.st.ccorner[c]);
+      p.st.ref[sc ^ 1][t] = q.x;
+      p.st.vel[sc ^ 1][t] = q.vv;
+      p.st.perm[sc ^ 1][t] = q.u;
+      p.st.cell[sc ^ 1][t] = c;
+      if (p.islot) p.islot[q.u] = t;
     };
     Pair nxt = load_pair(2 * gw);
     for (int cp = 2 * gw; cp < p.ncells; cp += 2 * nw) {
@@ -779,7 +801,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       if (!__any(len > 32)) {
         const bool mine = on && hl < len;
         const int v = cur.v;
-        const double zv = cur.zv;
+        const double zv = cur.x.z;
         // the half-wave's keys through the LDS (lane slots; the unused ones hold keys that
         // rank after every atom), read back as broadcasts up to the longer cell of the pair
         rz[tid] = zv;
@@ -794,7 +816,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
           r += (zx < zv || (zx == zv && vx < v)) ? 1 : 0;
         }
         __builtin_amdgcn_wave_barrier();
-        if (mine) gather(cm, a + r, v, cur.si);
+        if (mine) gather_pre(cm, a + r, cur);
       } else {
         for (int k = 0; k < 2; ++k) {
           const int c = cp + k;
@@ -827,11 +849,22 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       }
     }
   }
-  if (!grid_sync(p.bar, p.ints + 2)) return;
-  if (blockIdx.x == 0 && tid == 0) p.tphase[5] = wall_clock64();
-  // 5. bookkeeping: parity flips commit (drift, and the gather for rebuilt systems),
-  //    flags move to the list launch, rebuilt systems count one rebuild
-  if (blockIdx.x == 0) {
+  // 5. bookkeeping, by the last block to finish the gather (no grid barrier: no block reads
+  //    what the bookkeeping writes after its own gather, and the list launch sees it across
+  //    the launch boundary): parity flips commit (drift, and the gather for rebuilt
+  //    systems), flags move to the list launch, rebuilt systems count one rebuild. Every
+  //    parity and flag a block read has been consumed by its gather before it counts itself
+  //    done, so the flips cannot overtake a read.
+  __shared__ int last;
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned done = __hip_atomic_fetch_add(p.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == gridDim.x - 1;
+    if (last) __hip_atomic_store(p.bar + 2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (last) {
+    if (tid == 0) p.tphase[5] = wall_clock64();
     for (int s = tid; s < p.S; s += bd) {
       const int f = p.flags[s];
       p.par[s] ^= (s < p.flip_n ? 1 : 0) ^ (f ? 1 : 0);
@@ -1864,7 +1897,7 @@ struct BMD {
   std::vector<double> hc_len;
   std::vector<void*> cbuf;
   int* bflags = nullptr;   // rebuild flags handed to the list launch
-  unsigned* bar = nullptr;  // grid barrier of the rebuild launch (count, generation)
+  unsigned* bar = nullptr;  // rebuild launch: grid barrier count and generation, blocks done with the gather
   int* bsum = nullptr;      // per-block sums of the rebuild scan
   unsigned long long* look = nullptr;  // tagged per-block sums of the rebuild scan
   long long* tphase = nullptr;  // rebuild phase timestamps
