@@ -394,14 +394,15 @@ __device__ bool grid_sync(unsigned* __restrict__ bar, int* __restrict__ err) {
   __syncthreads();
   if (threadIdx.x == 0) {
     int good = 1;
-    // release this block's writes once; the polling loads below are relaxed so that the
-    // waiting blocks do not keep invalidating the caches of the working ones
+    // release this block's writes once (the fence makes the relaxed arrival below a release:
+    // each further release would be one more L2 write-back); the polling loads are relaxed
+    // so that the waiting blocks do not keep invalidating the caches of the working ones
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == gridDim.x - 1) {
       __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       int spins = 0;
       while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
@@ -415,6 +416,40 @@ __device__ bool grid_sync(unsigned* __restrict__ bar, int* __restrict__ err) {
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other blocks' writes in
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Grid-wide barrier for data exchanged only through agent-scope atomics: every wave waits
+// for its own memory operations (the atomics it issued are then performed), and the
+// arrival and the wait are relaxed, with no release or acquire fence, so no block writes
+// back or invalidates its L2. Plain stores made before it are NOT visible to other blocks
+// after it (a later grid_sync publishes them). Timeouts are reported as in grid_sync.
+__device__ bool grid_sync_atomics(unsigned* __restrict__ bar, int* __restrict__ err) {
+  __shared__ int ok;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int good = 1;
+    const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int spins = 0;
+      while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++spins > kBarSpin) {
+          good = 0;
+          atomicOr(err, 1);
+          atomicMax(err + 1, (int)__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
     ok = good;
   }
   __syncthreads();
@@ -646,9 +681,13 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       }
     }
   }
+  // cells of systems not rebuilt keep their counts (added to the zero the last scan left,
+  // as an atomic: the scan reads the counts across a barrier without fences)
   for (int c = gtid; c < p.ncells; c += gsz)
-    if (!p.flags[p.cellsys[c]]) p.count[c] = p.start[c + 1] - p.start[c];
-  if (!grid_sync(p.bar, p.ints + 2)) return;
+    if (!p.flags[p.cellsys[c]]) atomicAdd(&p.count[c], p.start[c + 1] - p.start[c]);
+  // the scan needs only the counts, all of them atomics: the binning's other stores (cells,
+  // slots, buckets) are published by the release of the barrier after the scan
+  if (!grid_sync_atomics(p.bar, p.ints + 2)) return;
   if (blockIdx.x == 0 && tid == 0) p.tphase[1] = wall_clock64();
   // 2. exclusive scan of the counts to the cell starts, counts back to 0, in one phase: each
   //    block sums its chunk of cells and publishes the sum in one 64-bit word tagged with
@@ -661,7 +700,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
         ((unsigned long long)(__hip_atomic_load(p.bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x7fffffffu) +
          1ull) << 32;
     int v = 0;
-    for (int c = c0 + tid; c < c1; c += bd) v += p.count[c];
+    for (int c = c0 + tid; c < c1; c += bd) v += __hip_atomic_load(&p.count[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     v = block_sum_int(v, wsum);
     if (tid == 0)
       __hip_atomic_store(&p.look[blockIdx.x], tag | (unsigned)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -692,7 +731,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     int off = s_off;
     for (int base = c0; base < c1; base += bd) {
       const int c = base + tid;
-      const int x = c < c1 ? p.count[c] : 0;
+      const int x = c < c1 ? __hip_atomic_load(&p.count[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
       int tot;
       const int ex = block_excl_scan(x, wsum, tot);
       if (c < c1) {
