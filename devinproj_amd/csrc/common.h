@@ -146,6 +146,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Deterministic sum of `nv` values per thread over the block; result valid in thread 0.
+// A wave's NV sums sit together (lds[w * NV + k]), so its stores take one base address
+// and immediate offsets (a kernel that sums inside a loop keeps one address live, not NV).
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV * nwaves] */) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -153,13 +155,13 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV * 
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
   if (lane == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) lds[k * nw + w] = v[k];
+    for (int k = 0; k < NV; ++k) lds[w * NV + k] = v[k];
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int q = 0; q < nw; ++q) s += lds[k * nw + q];
+      for (int q = 0; q < nw; ++q) s += lds[q * NV + k];
       v[k] = s;
     }
   }
