@@ -7,9 +7,10 @@ times whole MD steps on the wall clock (fused steps, records and the rebuilds th
 happen in the run), and returns the fastest as {"bricks": (bx, by, bz), "threads": t}.
 Results are cached per size bucket (atoms per cell, N, precision, interactions) in
 memory and optionally in a JSON file, so a batch of many systems tunes one
-representative per bucket (`tune_batch`). The list layout (16-byte groups of eight
-16-bit entries, one group row per `stride` atoms) is fixed, so the list stride is not
-a tuning parameter.
+representative per bucket (`tune_batch`). The list layout (16-byte groups of ten
+12-bit entries, or eight 16-bit ones when a brick region exceeds 4,095 slots; one
+group row per `stride` atoms) follows from the tile, so the list stride is not a
+tuning parameter.
 """
 from __future__ import annotations
 

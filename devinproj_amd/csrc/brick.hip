@@ -322,7 +322,7 @@ struct RB {
   int* bsum;    // [grid]
   unsigned long long* look;  // [grid] the scan's per-block sums, tagged with the barrier generation
   int* nn;
-  uint4* nbr;   // groups of 8 u16 local indices: group g of atom a at [g * stride + a]
+  uint4* nbr;   // 16-byte groups of local slot indices (12-bit x 10 or 16-bit x 8): group g of atom a at [g * stride + a]
   int stride, ngroups, cap;
   int* ints;    // [0] max neighbours, [1] region overflow, [2] barrier timeout, [3] blocks seen then,
                 // [4] a cell outgrew its bucket in this rebuild, [5] rebuilds in which one did
@@ -476,13 +476,17 @@ __device__ __forceinline__ const double4* cur_buf(const double4* A, const double
 }
 
 // Verlet list of one brick: every home atom against the region cells around its own
-// cell, 16-bit local indices in groups of 8, the last group padded with the index of
-// the far sentinel slot (`total`). The list only has to contain every pair within
+// cell, local slot indices in 16-byte groups, the last group padded with the index of
+// the far sentinel slot (`total`). W12: 12-bit indices, 10 per group, entry k at bits
+// 8 + 12 k (regions of at most 4,095 slots); else 16-bit indices, 8 per group, entry k
+// at bits 16 k. The list only has to contain every pair within
 // rc_list (a superset is harmless: forces apply their own cutoffs), so the test runs
 // in float32 on brick-relative coordinates with a 1e-5 relative margin on rc_list^2,
 // and region cells whose box is beyond rc_list from the atom are skipped.
-template <bool PK3>
+template <bool PK3, bool W12>
 __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs, int* hself, int* wsum, int& mymax) {
+  constexpr int kE = W12 ? 10 : 8;    // entries per group
+  constexpr int kW = W12 ? 12 : 16;   // bits per entry
   const BSys& s = p.sys[b.sys];
   const int total = region_setup(b, p.reg, p.start, rs, wsum, hself);
   if (total + 1 > p.cap) {
@@ -511,9 +515,10 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
     const int self = hself[local] + a;
     const float4 xi = ld_slot<PK3>(lds32, self);
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
-    // 8 entries per 16-byte group: each hit shifts the 128-bit register down by 16 bits
-    // and enters at the top, so the first entry of a group ends in the low half of w0
-    int cnt = 0, full = 7;  // full: the count at which the current group is complete
+    // each hit shifts the 128-bit register down by kW bits and enters at the top, so the
+    // first of a group's kE entries ends at bit 128 - kE kW (0 for 16-bit, 8 for 12-bit)
+    int cnt = 0, full = kE - 1;  // full: the count at which the current group is complete
+    int grp = 0;                 // groups stored
     unsigned w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     for (int ox = 0; ox < 3; ++ox) {
       const int rx = lx + ox;
@@ -543,13 +548,14 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
           for (int k = 0; k < 4; ++k) {
             const float dx = xj[k].x - xi.x, dy = xj[k].y - xi.y, dz = xj[k].z - xi.z;
             if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != skip) {
-              w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-              w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-              w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-              w3 = __builtin_amdgcn_alignbit((unsigned)(q + k), w3, 16);
-              if (cnt == full) {  // the eighth entry of a group: store the group
-                if ((cnt >> 3) < p.ngroups) p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
-                full += 8;
+              w0 = __builtin_amdgcn_alignbit(w1, w0, kW);
+              w1 = __builtin_amdgcn_alignbit(w2, w1, kW);
+              w2 = __builtin_amdgcn_alignbit(w3, w2, kW);
+              w3 = __builtin_amdgcn_alignbit((unsigned)(q + k), w3, kW);
+              if (cnt == full) {  // the last entry of a group: store the group
+                if (grp < p.ngroups) p.nbr[(size_t)grp * p.stride + a] = make_uint4(w0, w1, w2, w3);
+                ++grp;
+                full += kE;
               }
               ++cnt;
             }
@@ -557,16 +563,16 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
         }
       }
     }
-    if (cnt & 7) {  // pad the last group with the sentinel
-      for (int e = cnt & 7; e < 8; ++e) {
-        w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-        w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-        w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-        w3 = __builtin_amdgcn_alignbit(sentinel, w3, 16);
+    if (cnt > grp * kE) {  // pad the last group with the sentinel
+      for (int e = cnt - grp * kE; e < kE; ++e) {
+        w0 = __builtin_amdgcn_alignbit(w1, w0, kW);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, kW);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, kW);
+        w3 = __builtin_amdgcn_alignbit(sentinel, w3, kW);
       }
-      if ((cnt >> 3) < p.ngroups) p.nbr[(size_t)(cnt >> 3) * p.stride + a] = make_uint4(w0, w1, w2, w3);
+      if (grp < p.ngroups) p.nbr[(size_t)grp * p.stride + a] = make_uint4(w0, w1, w2, w3);
     }
-    const int capn = p.ngroups * 8;
+    const int capn = p.ngroups * kE;
     p.nn[a] = cnt < capn ? cnt : capn;
     mymax = cnt > mymax ? cnt : mymax;
   }
@@ -926,7 +932,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
 }
 
 // Verlet lists of the bricks of the systems the rebuild launch handed over.
-template <bool PK3>
+template <bool PK3, bool W12>
 __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   if (p.bflags[p.S] == 0 || p.ints[2] != 0) return;
   const BBrick b = p.bricks[blockIdx.x];
@@ -936,7 +942,7 @@ __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   __shared__ int wsum[kBT / 64];
   extern __shared__ __align__(16) float4 lds32[];
   int mymax = 0;
-  build_brick<PK3>(p, b, lds32, rs, hself, wsum, mymax);
+  build_brick<PK3, W12>(p, b, lds32, rs, hself, wsum, mymax);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
   if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&p.ints[0], mymax);
@@ -1046,7 +1052,7 @@ constexpr int fused_waves() {
   return sizeof(P) == 4 && LJ_ON && !COUL_ON && ONE_SP ? 6 : 1;
 }
 
-template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN>
+template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN, bool W12>
 __global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(fused_waves<P, LJ_ON, COUL_ON, ONE_SP>())))
 k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
                                                  const BRegion* __restrict__ reg, const int* __restrict__ start,
@@ -1145,27 +1151,49 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       ti = ONE_SP ? 0 : (int)xi.w;
       const int row = ti * s.ns;
       const P qi = (P)s.charge[ti];
-      const int ng = (nn[a] + 7) >> 3;
+      constexpr int kE = W12 ? 10 : 8;  // entries per 16-byte group of the list
+      const int ng = (nn[a] + kE - 1) / kE;
       P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
       unsigned ni = 0;  // pairs inside the cutoff (one species)
-      // groups of 8 neighbours; the next group's indices load while this one computes
+      // groups of kE neighbours; the next group's indices load while this one computes
       uint4 cur = make_uint4(0u, 0u, 0u, 0u);
       if (sub < ng) cur = nbr[(size_t)sub * stride + a];
       for (int g = sub; g < ng; g += gstep) {
         uint4 nxt = cur;
         if (g + gstep < ng) nxt = nbr[(size_t)(g + gstep) * stride + a];
-        const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
-#pragma unroll
-        for (int h2 = 0; h2 < 2; ++h2) {
-          const T4 x0 = ld_slot<PK3>(lds, w[2 * h2] & 0xffffu), x1 = ld_slot<PK3>(lds, w[2 * h2] >> 16);
-          const T4 x2 = ld_slot<PK3>(lds, w[2 * h2 + 1] & 0xffffu);
-          const T4 x3 = ld_slot<PK3>(lds, w[2 * h2 + 1] >> 16);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x0, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x1, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x2, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
-          b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, x3, fx, fy, fz, u, n, ni, cx, cy, cz, cu);
+        unsigned e[kE];
+        if constexpr (W12) {  // entry k at bits 8 + 12 k of the group
+          e[0] = (cur.x >> 8) & 0xfffu;
+          e[1] = cur.x >> 20;
+          e[2] = cur.y & 0xfffu;
+          e[3] = (cur.y >> 12) & 0xfffu;
+          e[4] = __builtin_amdgcn_alignbit(cur.z, cur.y, 24) & 0xfffu;
+          e[5] = (cur.z >> 4) & 0xfffu;
+          e[6] = (cur.z >> 16) & 0xfffu;
+          e[7] = __builtin_amdgcn_alignbit(cur.w, cur.z, 28) & 0xfffu;
+          e[8] = (cur.w >> 8) & 0xfffu;
+          e[9] = cur.w >> 20;
+        } else {  // entry k at bits 16 k
+          e[0] = cur.x & 0xffffu;
+          e[1] = cur.x >> 16;
+          e[2] = cur.y & 0xffffu;
+          e[3] = cur.y >> 16;
+          e[4] = cur.z & 0xffffu;
+          e[5] = cur.z >> 16;
+          e[6] = cur.w & 0xffffu;
+          e[7] = cur.w >> 16;
         }
-        if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every 8 neighbours
+        // half a group at a time: its slot reads are in flight together
+#pragma unroll
+        for (int h2 = 0; h2 < kE; h2 += kE / 2) {
+          T4 xj[kE / 2];
+#pragma unroll
+          for (int k = 0; k < kE / 2; ++k) xj[k] = ld_slot<PK3>(lds, e[h2 + k]);
+#pragma unroll
+          for (int k = 0; k < kE / 2; ++k)
+            b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, cx, cy, cz, cu);
+        }
+        if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every group
           Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
           fx = fy = fz = u = n = cx = cy = cz = cu = 0;
         }
@@ -1875,9 +1903,12 @@ template <typename P, typename F>
 void set_fused_attrs(F&& set) {
   auto one = [&](auto on, auto en) {
     constexpr bool O = decltype(on)::value, E = decltype(en)::value;
-    set((const void*)k_b_fused<P, true, true, O, E>);
-    set((const void*)k_b_fused<P, true, false, O, E>);
-    set((const void*)k_b_fused<P, false, true, O, E>);
+    set((const void*)k_b_fused<P, true, true, O, E, true>);
+    set((const void*)k_b_fused<P, true, false, O, E, true>);
+    set((const void*)k_b_fused<P, false, true, O, E, true>);
+    set((const void*)k_b_fused<P, true, true, O, E, false>);
+    set((const void*)k_b_fused<P, true, false, O, E, false>);
+    set((const void*)k_b_fused<P, false, true, O, E, false>);
   };
   one(std::true_type{}, std::true_type{});
   one(std::true_type{}, std::false_type{});
@@ -1920,8 +1951,12 @@ struct BMD {
   int* bucket = nullptr;  // per-cell arrival buckets of the rebuild, bcap atoms each
   int bcap = 0;
   int* nn = nullptr;
-  uint4* nbr = nullptr;  // groups of 8 u16 local indices, group g of atom a at [g * stride + a]
-  int stride = 0, max_nbrs = 0;  // max_nbrs: list capacity per atom, a multiple of 8
+  uint4* nbr = nullptr;  // 16-byte groups of slot indices (see w12), group g of atom a at [g * stride + a]
+  int stride = 0, max_nbrs = 0;  // max_nbrs: list capacity per atom, a multiple of the entries per group
+  // list entries: 12-bit slot indices, 10 per 16-byte group, when every region fits 4,095
+  // slots (a fifth less list traffic per step); else 16-bit, 8 per group
+  bool w12 = false;
+  int epg() const { return w12 ? 10 : 8; }
   int* flags = nullptr;  // [S] per system + [S] any
   int* ints = nullptr;   // [0] max neighbours, [1] region overflow, [2] rebuild barrier timeout
   int* nreb = nullptr;
@@ -1996,6 +2031,11 @@ struct BMD {
   // the list launch stages 12-byte slots when 16-byte ones would keep fewer than 4 bricks
   // of 512 threads on a CU (160 KiB of LDS); 16-byte slots read in one instruction
   bool pk3_build() const { return (size_t)cap_build * sizeof(float4) > 36 * 1024; }
+  // the list launch of this engine's slot size and list format
+  void (*build_kernel() const)(RB) {
+    return pk3_build() ? (w12 ? k_b_build<true, true> : k_b_build<true, false>)
+                       : (w12 ? k_b_build<false, true> : k_b_build<false, false>);
+  }
   // (4 slots past the capacity: the list build reads up to 3 slots beyond a column's last
   // one and discards them, so its candidate addresses need no clamp)
   size_t lds_build() const {
@@ -2027,7 +2067,7 @@ struct BMD {
     p.nn = nn;
     p.nbr = nbr;
     p.stride = stride;
-    p.ngroups = max_nbrs / 8;
+    p.ngroups = max_nbrs / epg();
     p.cap = cap_build;
     p.ints = ints;
     p.tphase = tphase;
@@ -2069,7 +2109,7 @@ struct BMD {
   void launch_rebuild(int flip_n, int rec = 0) {
     const RB p = rebuild_args(0, flip_n, rec);
     hipLaunchKernelGGL(k_b_rebuild, dim3(grid_rb), dim3(kRB), 0, st, p);
-    hipLaunchKernelGGL(pk3_build() ? k_b_build<true> : k_b_build<false>, dim3(nbricks), dim3(build_threads()),
+    hipLaunchKernelGGL(build_kernel(), dim3(nbricks), dim3(build_threads()),
                        lds_build(), st, p);
   }
 
@@ -2082,9 +2122,15 @@ struct BMD {
                          spar, nn, nbr, stride, cap_fused, wc, wn, drift_from, flags, S, partials, ints, sn,
                          snap_cap, snap_count, N);
     };
-    if (lj && co) go(k_b_fused<P, true, true, ONE, EN>);
-    else if (lj) go(k_b_fused<P, true, false, ONE, EN>);
-    else go(k_b_fused<P, false, true, ONE, EN>);
+    if (w12) {
+      if (lj && co) go(k_b_fused<P, true, true, ONE, EN, true>);
+      else if (lj) go(k_b_fused<P, true, false, ONE, EN, true>);
+      else go(k_b_fused<P, false, true, ONE, EN, true>);
+    } else {
+      if (lj && co) go(k_b_fused<P, true, true, ONE, EN, false>);
+      else if (lj) go(k_b_fused<P, true, false, ONE, EN, false>);
+      else go(k_b_fused<P, false, true, ONE, EN, false>);
+    }
   }
 
   bool any_lj = false, any_coul = false, one_sp = true;
@@ -2607,10 +2653,11 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   m.cap_fused = cap;
   ARG_CHECK(m.lds_build() + 6144 <= lds_max && m.lds_fused() + 6144 <= lds_max && cap < 65536,
             "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
-  (void)hipFuncSetAttribute(m.pk3_build() ? (const void*)k_b_build<true> : (const void*)k_b_build<false>,
+  m.w12 = cap <= 4095;  // the sentinel slot (index < cap) included
+  (void)hipFuncSetAttribute((const void*)m.build_kernel(),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
   // neighbour capacity: measure with one group of 8, then size
-  m.max_nbrs = 8;
+  m.max_nbrs = m.epg();
   HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * sizeof(uint4)));
   hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, m.st, m.flags, m.S + 1, 1);
   m.launch_rebuild(0);
@@ -2619,9 +2666,9 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   HIP_TRY(hipStreamSynchronize(m.st));
   ARG_CHECK(ints[2] == 0, "rebuild grid barrier timed out at set-up: %d of %d blocks arrived", ints[3], m.grid_rb);
   ARG_CHECK(ints[1] == 0, "region overflow at the first build (%d atoms)", ints[1]);
-  const int nbrs = ((int)(ints[0] * (1.0 + slack)) + 16 + 7) / 8 * 8;
+  const int nbrs = ((int)(ints[0] * (1.0 + slack)) + 16 + m.epg() - 1) / m.epg() * m.epg();
   HIP_TRY(hipFree(m.nbr));
-  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * (nbrs / 8) * sizeof(uint4)));
+  HIP_TRY(hipMalloc(&m.nbr, (size_t)m.stride * (nbrs / m.epg()) * sizeof(uint4)));
   m.max_nbrs = nbrs;
   hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, m.st, m.flags, m.S + 1, 1);
   m.launch_rebuild(0);
