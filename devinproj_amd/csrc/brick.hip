@@ -54,6 +54,7 @@ struct BSys {
   CoulParams coul;
   double rc_list2;
   double half_skin2;
+  double half_skin2_q;  // the same in fixed-point quanta squared (mixed mode)
   double dt;
   // mixed mode: the fixed-point position copy (offsets from each atom's cell corner in
   // quanta of unit * 2^-fx_k, `unit` being the staging unit of the fused launch)
@@ -247,6 +248,9 @@ struct BState {
   // each atom's cell, species in w; posf[0] pairs with posA and posf[1] with posB (the
   // position parity, not the state parity)
   int4* posf[2];
+  // mixed mode only (else null): the fixed-point positions at the last list build, in the
+  // layout of posf and the state parity of ref (which mixed mode leaves unused)
+  int4* refq[2];
   const double4* ccorner;  // corner of every cell in the box frame (both modes)
 };
 
@@ -262,6 +266,15 @@ __device__ __forceinline__ int4 fix_pos(const BSys& s, const double4& x, const d
     q[d] = (int)rint(e * s.fx_q);
   }
   return make_int4(q[0], q[1], q[2], (int)x.w);
+}
+
+// Mixed mode: whether an atom has moved more than half the skin since the last list
+// build, from its fixed-point position now (f) and then (r). Both are offsets from the
+// corner of the same cell (cells change only at rebuilds), so the displacement is an
+// exact integer difference, compared in quanta squared.
+__device__ __forceinline__ bool moved_far_q(const BSys& s, const int4& f, const int4& r) {
+  const double dx = (double)(f.x - r.x), dy = (double)(f.y - r.y), dz = (double)(f.z - r.z);
+  return !(fma(dz, dz, fma(dy, dy, dx * dx)) < s.half_skin2_q);
 }
 
 // Mixed mode: stage the region from the fixed-point copy. Region cell r (an image of cell
@@ -782,8 +795,13 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       const int pc = rb_par(p, si), sc = p.spar[si];
       const double4 x = (pc ? p.posB : p.posA)[v];
       (pc ? p.posA : p.posB)[t] = x;
-      if (p.st.posf[0]) p.st.posf[pc ^ 1][t] = fix_pos(p.sys[si], x, p.st.ccorner[c]);
-      p.st.ref[sc ^ 1][t] = x;
+      if (p.st.posf[0]) {
+        const int4 f = fix_pos(p.sys[si], x, p.st.ccorner[c]);
+        p.st.posf[pc ^ 1][t] = f;
+        p.st.refq[sc ^ 1][t] = f;
+      } else {
+        p.st.ref[sc ^ 1][t] = x;
+      }
       p.st.vel[sc ^ 1][t] = p.st.vel[sc][v];
       const int u = p.st.perm[sc][v];
       p.st.perm[sc ^ 1][t] = u;
@@ -829,8 +847,13 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
     auto gather_pre = [&](int c, int t, const Pair& q) {
       const int pc = rb_par(p, q.si), sc = p.spar[q.si];
       (pc ? p.posA : p.posB)[t] = q.x;
-      if (p.st.posf[0]) p.st.posf[pc ^ 1][t] = fix_pos(p.sys[q.si], q.x, p.st.ccorner[c]);
-      p.st.ref[sc ^ 1][t] = q.x;
+      if (p.st.posf[0]) {
+        const int4 f = fix_pos(p.sys[q.si], q.x, p.st.ccorner[c]);
+        p.st.posf[pc ^ 1][t] = f;
+        p.st.refq[sc ^ 1][t] = f;
+      } else {
+        p.st.ref[sc ^ 1][t] = q.x;
+      }
       p.st.vel[sc ^ 1][t] = q.vv;
       p.st.perm[sc ^ 1][t] = q.u;
       p.st.cell[sc ^ 1][t] = c;
@@ -1290,12 +1313,17 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
         x.y = np_mod2(x.y, s.L[1]);
         x.z = np_mod2(x.z, s.L[2]);
         pos_next[a] = x;
-        if constexpr (sizeof(P) == 4) sta.posf[par[si] ^ 1][a] = fix_pos(s, x, sta.ccorner[cell[a]]);
-        const double4 r = ref[a];
-        const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
-        const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
-        const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
-        moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+        if constexpr (sizeof(P) == 4) {
+          const int4 f = fix_pos(s, x, sta.ccorner[cell[a]]);
+          sta.posf[par[si] ^ 1][a] = f;
+          moved_far |= moved_far_q(s, f, sta.refq[sp][a]);
+        } else {
+          const double4 r = ref[a];
+          const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+          const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+          const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+          moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+        }
       }
       vel[a] = v;
       }
@@ -1450,12 +1478,17 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
       x.y = np_mod2(x.y, s.L[1]);
       x.z = np_mod2(x.z, s.L[2]);
       pos_next[a] = x;
-      if (sta.posf[0]) sta.posf[par[si] ^ 1][a] = fix_pos(s, x, sta.ccorner[sta.cell[sp][a]]);
-      const double4 r = ref[a];
-      const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
-      const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
-      const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
-      moved_far = !(r2_np(d0, d1, d2) < s.half_skin2);
+      if (sta.posf[0]) {
+        const int4 f = fix_pos(s, x, sta.ccorner[sta.cell[sp][a]]);
+        sta.posf[par[si] ^ 1][a] = f;
+        moved_far = moved_far_q(s, f, sta.refq[sp][a]);
+      } else {
+        const double4 r = ref[a];
+        const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+        const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+        const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+        moved_far = !(r2_np(d0, d1, d2) < s.half_skin2);
+      }
       if (moved_far) {
         atomicOr(&flags[si], 1);
         atomicOr(&flags[S], 1);
@@ -1767,16 +1800,21 @@ __device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restric
     v.y = v.y + D(5 * j + 1) / s.dt;
     v.z = v.z + D(5 * j + 2) / s.dt;
     nw[t] = x;
-    if (sta.posf[0]) sta.posf[par[si] ^ 1][t] = fix_pos(s, x, sta.ccorner[sta.cell[sp][t]]);
     vel[t] = v;
     D(5 * j) = x.x;  // the displacement is applied: keep the final position for the check below
     D(5 * j + 1) = x.y;
     D(5 * j + 2) = x.z;
-    const double4 r = ref[t];
-    const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
-    const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
-    const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
-    far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+    if (sta.posf[0]) {
+      const int4 f = fix_pos(s, x, sta.ccorner[sta.cell[sp][t]]);
+      sta.posf[par[si] ^ 1][t] = f;
+      far |= moved_far_q(s, f, sta.refq[sp][t]);
+    } else {
+      const double4 r = ref[t];
+      const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
+      const double d1 = minimg_fast(x.y - r.y, s.L[1], s.inv[1]);
+      const double d2 = minimg_fast(x.z - r.z, s.L[2], s.inv[2]);
+      far |= !(r2_np(d0, d1, d2) < s.half_skin2);
+    }
   }
   if (far) {
     atomicOr(&flags[si], 1);
@@ -2012,7 +2050,7 @@ struct BMD {
                   sta.vel[1], sta.F[0], sta.F[1], sta.ref[0], sta.ref[1], sta.perm[0], sta.perm[1], sta.cell[0],
                   sta.cell[1], count, start, tmp, slot, bucket, nn, nbr, flags, ints, nreb, log_count, log, partials, bflags,
                   bar, bsum, look, tphase, snap, snap_count, snap_rec, ndrift, nh, sta.posf[0], sta.posf[1],
-                  (void*)sta.ccorner};
+                  sta.refq[0], sta.refq[1], (void*)sta.ccorner};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     for (void* p : cbuf) (void)hipFree(p);
@@ -2330,6 +2368,7 @@ static int build_tables(BMD& m) {
       ARG_CHECK(k > -100 && k < 100, "system %d: no fixed-point scale for cells of %.3g", si, s.hc[0]);
       s.fx_k = k;
       s.fx_q = std::ldexp(1.0, k) / unit;
+      s.half_skin2_q = s.half_skin2 > 0.0 ? s.half_skin2 * s.fx_q * s.fx_q : -1.0;
       s.fx_s = std::ldexp(1.0f, -k);
       s.fx_sm = (float)(unit * std::ldexp(1.0, -k));
     }
@@ -2451,7 +2490,10 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
                   (void**)&m->tmp,         (void**)&m->slot,        (void**)&m->nn};
   for (void** p : i4s) HIP_TRY(hipMalloc(p, i4 + 64));
   if (precision == ACCU_FP32_PAIR)
-    for (int k = 0; k < 2; ++k) HIP_TRY(hipMalloc(&m->sta.posf[k], (size_t)N * sizeof(int4) + 64));
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(hipMalloc(&m->sta.posf[k], (size_t)N * sizeof(int4) + 64));
+      HIP_TRY(hipMalloc(&m->sta.refq[k], (size_t)N * sizeof(int4) + 64));
+    }
   HIP_TRY(hipMalloc(&m->par, nsys * sizeof(int) + 64));
   HIP_TRY(hipMemset(m->par, 0, nsys * sizeof(int) + 64));
   HIP_TRY(hipMalloc(&m->spar, nsys * sizeof(int) + 64));
