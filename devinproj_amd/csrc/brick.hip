@@ -126,6 +126,30 @@ __device__ __forceinline__ void st_slot(T4* lds, unsigned q, const T4& v) {
   }
 }
 
+// Entry k of a 16-byte list group (k a compile-time constant after unrolling): W12, 12-bit
+// slot indices at bits 8 + 12 k (10 per group); else 16-bit at bits 16 k (8 per group).
+// Decoded where the slot is read, so only the group itself stays live.
+template <bool W12>
+__device__ __forceinline__ unsigned list_entry(const uint4& g, int k) {
+  if constexpr (W12) {
+    switch (k) {
+      case 0: return (g.x >> 8) & 0xfffu;
+      case 1: return g.x >> 20;
+      case 2: return g.y & 0xfffu;
+      case 3: return (g.y >> 12) & 0xfffu;
+      case 4: return __builtin_amdgcn_alignbit(g.z, g.y, 24) & 0xfffu;
+      case 5: return (g.z >> 4) & 0xfffu;
+      case 6: return (g.z >> 16) & 0xfffu;
+      case 7: return __builtin_amdgcn_alignbit(g.w, g.z, 28) & 0xfffu;
+      case 8: return (g.w >> 8) & 0xfffu;
+      default: return g.w >> 20;
+    }
+  } else {
+    const unsigned w = k < 2 ? g.x : k < 4 ? g.y : k < 6 ? g.z : g.w;
+    return (k & 1) ? (w >> 16) : (w & 0xffffu);
+  }
+}
+
 // Exclusive prefix of v over the block; total receives the block sum.
 __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -1184,36 +1208,15 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       for (int g = sub; g < ng; g += gstep) {
         uint4 nxt = cur;
         if (g + gstep < ng) nxt = nbr[(size_t)(g + gstep) * stride + a];
-        unsigned e[kE];
-        if constexpr (W12) {  // entry k at bits 8 + 12 k of the group
-          e[0] = (cur.x >> 8) & 0xfffu;
-          e[1] = cur.x >> 20;
-          e[2] = cur.y & 0xfffu;
-          e[3] = (cur.y >> 12) & 0xfffu;
-          e[4] = __builtin_amdgcn_alignbit(cur.z, cur.y, 24) & 0xfffu;
-          e[5] = (cur.z >> 4) & 0xfffu;
-          e[6] = (cur.z >> 16) & 0xfffu;
-          e[7] = __builtin_amdgcn_alignbit(cur.w, cur.z, 28) & 0xfffu;
-          e[8] = (cur.w >> 8) & 0xfffu;
-          e[9] = cur.w >> 20;
-        } else {  // entry k at bits 16 k
-          e[0] = cur.x & 0xffffu;
-          e[1] = cur.x >> 16;
-          e[2] = cur.y & 0xffffu;
-          e[3] = cur.y >> 16;
-          e[4] = cur.z & 0xffffu;
-          e[5] = cur.z >> 16;
-          e[6] = cur.w & 0xffffu;
-          e[7] = cur.w >> 16;
-        }
         // half a group at a time: its slot reads are in flight together
+        constexpr int kC = kE / 2;
 #pragma unroll
-        for (int h2 = 0; h2 < kE; h2 += kE / 2) {
-          T4 xj[kE / 2];
+        for (int h2 = 0; h2 < kE; h2 += kC) {
+          T4 xj[kC];
 #pragma unroll
-          for (int k = 0; k < kE / 2; ++k) xj[k] = ld_slot<PK3>(lds, e[h2 + k]);
+          for (int k = 0; k < kC; ++k) xj[k] = ld_slot<PK3>(lds, list_entry<W12>(cur, h2 + k));
 #pragma unroll
-          for (int k = 0; k < kE / 2; ++k)
+          for (int k = 0; k < kC; ++k)
             b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, cx, cy, cz, cu);
         }
         if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every group
@@ -2695,7 +2698,10 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   m.cap_fused = cap;
   ARG_CHECK(m.lds_build() + 6144 <= lds_max && m.lds_fused() + 6144 <= lds_max && cap < 65536,
             "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
-  m.w12 = cap <= 4095;  // the sentinel slot (index < cap) included
+  // 12-bit entries need every slot index (the sentinel's included) below 4,096; the float64
+  // Coulomb step keeps 16-bit ones (ten of its pairs per group need more than 256 VGPRs and
+  // spill inside the pair loop, where eight spill a handful)
+  m.w12 = cap <= 4095 && !(m.precision == ACCU_FP64 && m.any_coul);
   (void)hipFuncSetAttribute((const void*)m.build_kernel(),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
   // neighbour capacity: measure with one group of 8, then size
