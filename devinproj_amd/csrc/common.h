@@ -145,11 +145,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Deterministic sum of `nv` values per thread over the block; result valid in thread 0.
+// Deterministic sum of `nv` values per thread over the block; result valid in wave 0.
 // A wave's NV sums sit together (lds[w * NV + k]), so its stores take one base address
 // and immediate offsets (a kernel that sums inside a loop keeps one address live, not NV).
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV * nwaves] */) {
+  static_assert(NV <= 64, "one lane of wave 0 per value");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -157,12 +158,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV * 
 #pragma unroll
     for (int k = 0; k < NV; ++k) lds[w * NV + k] = v[k];
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {  // wave 0: lane k sums value k over the waves, in wave order
+    double s = 0.0;
+    if (lane < NV)
+      for (int q = 0; q < nw; ++q) s += lds[q * NV + lane];
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, s);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int q = 0; q < nw; ++q) s += lds[q * NV + k];
-      v[k] = s;
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+      v[k] = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
     }
   }
   __syncthreads();
