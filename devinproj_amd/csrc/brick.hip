@@ -60,6 +60,13 @@ struct BSys {
   // quanta of unit * 2^-fx_k, `unit` being the staging unit of the fused launch)
   double hc[3];    // cell edge L / nc
   double fx_q;     // 2^fx_k / unit: metres -> quanta
+  // the list launch's integer staging: coordinates in quanta bq = 2^bq_sh / fx_q (about 1/8192
+  // of a cell), x and y as int16, z as int32; a pair is listed when its squared quantised
+  // distance is below bq_T, which covers rc_list plus the rounding of both atoms
+  double bq_inv;   // 1 / bq (metres -> build quanta)
+  int bq_sh, bq_T, bq_W;  // shift from the fixed-point quanta; r^2 threshold; z window half-width
+  float bq_m;      // bq in metres
+  float bq_rcp2;   // squared radius of the column prune (metres), rc_list widened by the rounding
   float fx_s;      // 2^-fx_k: quanta -> staging units of the fused launch (exact)
   float fx_sm;     // unit * 2^-fx_k: quanta -> metres (the list launch)
   int fx_k;
@@ -337,6 +344,59 @@ __device__ void stage_region_fix(const BBrick& b, const BSys& s, const BRegion* 
   }
 }
 
+// The list launch's region staging: 8-byte integer slots (x, y as int16 in the low and
+// high halves of the first word, z as int32 in the second), in build quanta relative to the
+// brick centre. FIX (mixed mode): from the fixed-point copy, the exact integer coordinate
+// rounded by an arithmetic shift; else from the float64 positions with the image of
+// stage_region. Rounding is monotone, so each column's run stays sorted by z.
+__device__ __forceinline__ uint2 pack_q(int x, int y, int z) {
+  return make_uint2(((unsigned)x & 0xffffu) | ((unsigned)y << 16), (unsigned)z);
+}
+template <bool FIX>
+__device__ void stage_region_q(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
+                               const int* __restrict__ start, const int* rs, const int4* __restrict__ posf,
+                               const double4* __restrict__ pos, const double4* __restrict__ ccorner, uint2* lds) {
+  const int R = b.rd[0] * b.rd[1] * b.rd[2];
+  const int G = max(1, (int)blockDim.x / R);  // threads per region cell
+  const int per = (int)blockDim.x / G;         // region cells per pass
+  const int sub = threadIdx.x % G;
+  const int sh = s.bq_sh, half = sh > 0 ? 1 << (sh - 1) : 0;
+  for (int r = threadIdx.x / G; r < R && threadIdx.x < per * G; r += per) {
+    const int q0 = rs[r], q1 = rs[r + 1];
+    const BRegion rg = reg[b.reg_off + r];
+    const int base = start[rg.gid] - q0;
+    if constexpr (FIX) {
+      const double4 cc = ccorner[rg.gid];
+      const double c3[3] = {cc.x, cc.y, cc.z};
+      const int shv[3] = {rg.shift % 3 - 1, (rg.shift / 3) % 3 - 1, rg.shift / 9 - 1};
+      int B[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) B[d] = (int)rint((c3[d] + shv[d] * s.L[d] - b.org[d]) * s.fx_q);
+      for (int q = q0 + sub; q < q1; q += G) {
+        const int4 o = posf[base + q];
+        lds[q] = pack_q((B[0] + o.x + half) >> sh, (B[1] + o.y + half) >> sh, (B[2] + o.z + half) >> sh);
+      }
+    } else {
+      const int rr[3] = {r / (b.rd[2] * b.rd[1]), (r / b.rd[2]) % b.rd[1], r % b.rd[2]};
+      double cen[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)  // region cell 0 starts 1 + hd/2 cells below the brick centre
+        cen[d] = (rr[d] - 0.5 - 0.5 * b.hd[d]) * (s.L[d] / s.nc[d]);
+      for (int q = q0 + sub; q < q1; q += G) {
+        const double4 x = pos[base + q];
+        const double xs[3] = {x.x, x.y, x.z};
+        int X[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double e = xs[d] - b.org[d];
+          X[d] = (int)rint(fma(-s.L[d], rint((e - cen[d]) * s.inv[d]), e) * s.bq_inv);
+        }
+        lds[q] = pack_q(X[0], X[1], X[2]);
+      }
+    }
+  }
+}
+
 struct RB {
   const BSys* sys;
   const BBrick* bricks;
@@ -514,14 +574,16 @@ __device__ __forceinline__ const double4* cur_buf(const double4* A, const double
 
 // Verlet list of one brick: every home atom against the region cells around its own
 // cell, local slot indices in 16-byte groups, the last group padded with the index of
-// the far sentinel slot (`total`). W12: 12-bit indices, 10 per group, entry k at bits
-// 8 + 12 k (regions of at most 4,095 slots); else 16-bit indices, 8 per group, entry k
-// at bits 16 k. The list only has to contain every pair within
-// rc_list (a superset is harmless: forces apply their own cutoffs), so the test runs
-// in float32 on brick-relative coordinates with a 1e-5 relative margin on rc_list^2,
-// and region cells whose box is beyond rc_list from the atom are skipped.
-template <bool PK3, bool W12>
-__device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs, int* hself, int* wsum, int& mymax) {
+// the far sentinel slot (`total`). W12: 12-bit slot indices, 10 per group, entry k at
+// bits 8 + 12 k (regions of at most 4,095 slots); else 16-bit, 8 per group, entry k at
+// bits 16 k. The list only has to contain every pair within rc_list (a superset is
+// harmless: forces apply their own cutoffs), so the test runs on integer coordinates
+// (stage_region_q): a packed 16-bit difference of x and y, an int32 difference of z,
+// one dot product, against a threshold that covers rc_list plus the rounding of both
+// atoms; region cells whose box is beyond reach of the atom are skipped.
+typedef short short2v __attribute__((ext_vector_type(2)));
+template <bool W12>
+__device__ void build_brick(const RB& p, const BBrick& b, uint2* lds, int* rs, int* hself, int* wsum, int& mymax) {
   constexpr int kE = W12 ? 10 : 8;    // entries per group
   constexpr int kW = W12 ? 12 : 16;   // bits per entry
   const BSys& s = p.sys[b.sys];
@@ -530,27 +592,27 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
     if (threadIdx.x == 0) atomicMax(&p.ints[1], total + 1);
     return;
   }
-  // PK3: 12-byte slots (the list test needs no species), so that more bricks share a CU's LDS;
-  // mixed mode stages from the fixed-point copy (in metres: the margins below cover the
-  // one extra rounding of the scale)
   if (p.st.posf[0])
-    stage_region_fix<PK3>(b, s, p.reg, p.start, rs, p.st.posf[p.par[b.sys]], p.st.ccorner, lds32, s.fx_sm);
+    stage_region_q<true>(b, s, p.reg, p.start, rs, p.st.posf[p.par[b.sys]], nullptr, p.st.ccorner, lds);
   else
-    stage_region<float, PK3>(b, s, p.reg, p.start, rs, cur_buf(p.posA, p.posB, p.par, b.sys), lds32);
+    stage_region_q<false>(b, s, p.reg, p.start, rs, nullptr, cur_buf(p.posA, p.posB, p.par, b.sys), nullptr, lds);
   __syncthreads();
   const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
                        -(1.0f + 0.5f * b.hd[2]) * h[2]};  // low corner of region cell 0
-  const float rc2 = (float)(s.rc_list2 * (1.0 + 1e-5));
-  // z window half-width: a candidate with dz <= -rcw or dz >= rcw has dz^2 >= rc2 and fails the test anyway
-  const float rcw = sqrtf(rc2) * (1.0f + 1e-6f);
+  const float rcp2 = s.bq_rcp2, bqm = s.bq_m;
+  // a candidate with |dz| >= W (quanta) has dz^2 >= T and fails the test anyway
+  const int T = s.bq_T, W = s.bq_W;
   const unsigned sentinel = (unsigned)total;
   const int* __restrict__ cell = p.st.cell[p.spar[b.sys]];
   const int a0 = p.start[b.cell0], a1 = p.start[b.cell0 + b.nhome];
   for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
     const int local = cell[a] - b.cell0;
     const int self = hself[local] + a;
-    const float4 xi = ld_slot<PK3>(lds32, self);
+    const uint2 si = lds[self];
+    const short2v xyi = __builtin_bit_cast(short2v, si.x);
+    const int zi = (int)si.y;
+    const float xm = (float)xyi.x * bqm, ym = (float)xyi.y * bqm;  // metres, for the column prune
     const int lz = local % b.hd[2], ly = (local / b.hd[2]) % b.hd[1], lx = local / (b.hd[2] * b.hd[1]);
     // each hit shifts the 128-bit register down by kW bits and enters at the top, so the
     // first of a group's kE entries ends at bit 128 - kE kW (0 for 16-bit, 8 for 12-bit)
@@ -559,32 +621,41 @@ __device__ void build_brick(const RB& p, const BBrick& b, float4* lds32, int* rs
     unsigned w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     for (int ox = 0; ox < 3; ++ox) {
       const int rx = lx + ox;
-      const float ex = fmaxf(fmaxf(c0[0] + rx * h[0] - xi.x, xi.x - (c0[0] + (rx + 1) * h[0])), 0.f);
+      const float ex = fmaxf(fmaxf(c0[0] + rx * h[0] - xm, xm - (c0[0] + (rx + 1) * h[0])), 0.f);
       for (int oy = 0; oy < 3; ++oy) {
         const int ry = ly + oy;
-        const float ey = fmaxf(fmaxf(c0[1] + ry * h[1] - xi.y, xi.y - (c0[1] + (ry + 1) * h[1])), 0.f);
-        if (ex * ex + ey * ey > rc2) continue;  // the column's box is out of range
+        const float ey = fmaxf(fmaxf(c0[1] + ry * h[1] - ym, ym - (c0[1] + (ry + 1) * h[1])), 0.f);
+        if (ex * ex + ey * ey > rcp2) continue;  // the column's box is out of reach
         // the column's three cells (rz = lz .. lz + 2) are one run of slots sorted by z:
-        // scan only from near the first candidate with dz > -rcw (skipping 8 slots at a
-        // time past those below) to the first with dz >= rcw
+        // scan only from near the first candidate with dz > -W (skipping 8 slots at a
+        // time past those below) to the first with dz >= W
         const int r0 = (rx * b.rd[1] + ry) * b.rd[2] + lz;
         int lo = rs[r0];
         const int qe = rs[r0 + 3];
-        while (lo + 8 < qe && ld_slot<PK3>(lds32, lo + 8).z - xi.z <= -rcw) lo += 8;
+        while (lo + 8 < qe && (int)lds[lo + 8].y - zi <= -W) lo += 8;
         // the atom itself is a candidate only in its own column
         const int skip = (ox == 1 && oy == 1) ? self : -1;
-        // four candidates per step: their LDS reads are in flight together, at one base
-        // address plus fixed offsets (the slots past the run's end, at most 3 of them and
-        // inside the LDS padding, are read and rejected by q < qe)
-        for (int q = lo; q < qe; q += 4) {
-          float4 xj[4];
+        // four candidates per step, at one base address plus fixed offsets; the next
+        // step's four are read while this step's are tested, so no read waits on the loop
+        // test (slots past the run's end, at most 7 of them and inside the LDS padding,
+        // are read and either rejected by q + k < qe or never used)
+        uint2 nx[4];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) xj[k] = ld_slot<PK3>(lds32, q + k);
-          if (xj[0].z - xi.z >= rcw) break;  // this and every later candidate is above the window
+        for (int k = 0; k < 4; ++k) nx[k] = lds[lo + k];
+        for (int q = lo; q < qe; q += 4) {
+          uint2 xj[4];
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            const float dx = xj[k].x - xi.x, dy = xj[k].y - xi.y, dz = xj[k].z - xi.z;
-            if (dx * dx + dy * dy + dz * dz < rc2 && q + k < qe && q + k != skip) {
+            xj[k] = nx[k];
+            nx[k] = lds[q + 4 + k];
+          }
+          if ((int)xj[0].y - zi >= W) break;  // this and every later candidate is above the window
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const short2v dxy = __builtin_bit_cast(short2v, xj[k].x) - xyi;
+            const int dz = (int)xj[k].y - zi;
+            const int r2 = __builtin_amdgcn_sdot2(dxy, dxy, __mul24(dz, dz), false);
+            if (r2 < T && q + k < qe && q + k != skip) {
               w0 = __builtin_amdgcn_alignbit(w1, w0, kW);
               w1 = __builtin_amdgcn_alignbit(w2, w1, kW);
               w2 = __builtin_amdgcn_alignbit(w3, w2, kW);
@@ -979,7 +1050,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
 }
 
 // Verlet lists of the bricks of the systems the rebuild launch handed over.
-template <bool PK3, bool W12>
+template <bool W12>
 __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   if (p.bflags[p.S] == 0 || p.ints[2] != 0) return;
   const BBrick b = p.bricks[blockIdx.x];
@@ -987,9 +1058,9 @@ __global__ void __launch_bounds__(kBT) k_b_build(RB p) {
   __shared__ int rs[kMaxRegionCells + 1];
   __shared__ int hself[kMaxHomeCells];
   __shared__ int wsum[kBT / 64];
-  extern __shared__ __align__(16) float4 lds32[];
+  extern __shared__ __align__(16) uint2 ldsq[];
   int mymax = 0;
-  build_brick<PK3, W12>(p, b, lds32, rs, hself, wsum, mymax);
+  build_brick<W12>(p, b, ldsq, rs, hself, wsum, mymax);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, __shfl_down(mymax, o, 64));
   if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&p.ints[0], mymax);
@@ -2069,20 +2140,12 @@ struct BMD {
                         : (any_lj && !any_coul && one_sp) ? sizeof(Pk3) : sizeof(float4);
     return ((size_t)cap_fused * slot + 15) / 16 * 16;
   }
-  // the list launch stages 12-byte slots when 16-byte ones would keep fewer than 4 bricks
-  // of 512 threads on a CU (160 KiB of LDS); 16-byte slots read in one instruction
-  bool pk3_build() const { return (size_t)cap_build * sizeof(float4) > 36 * 1024; }
-  // the list launch of this engine's slot size and list format
-  void (*build_kernel() const)(RB) {
-    return pk3_build() ? (w12 ? k_b_build<true, true> : k_b_build<true, false>)
-                       : (w12 ? k_b_build<false, true> : k_b_build<false, false>);
-  }
-  // (4 slots past the capacity: the list build reads up to 3 slots beyond a column's last
-  // one and discards them, so its candidate addresses need no clamp)
-  size_t lds_build() const {
-    return pk3_build() ? ((size_t)(cap_build + 4) * sizeof(Pk3) + 15) / 16 * 16
-                       : (size_t)(cap_build + 4) * sizeof(float4);
-  }
+  // the list launch of this engine's list format
+  void (*build_kernel() const)(RB) { return w12 ? k_b_build<true> : k_b_build<false>; }
+  // the list launch stages 8-byte integer slots (stage_region_q), 8 past the capacity: it
+  // reads up to 7 slots beyond a column's last one and discards them, so its candidate
+  // addresses need no clamp
+  size_t lds_build() const { return (size_t)(cap_build + 8) * sizeof(uint2); }
 
   RB rebuild_args(int bin_only, int flip_n, int rec = 0) const {
     RB p;
@@ -2374,6 +2437,27 @@ static int build_tables(BMD& m) {
       s.half_skin2_q = s.half_skin2 > 0.0 ? s.half_skin2 * s.fx_q * s.fx_q : -1.0;
       s.fx_s = std::ldexp(1.0f, -k);
       s.fx_sm = (float)(unit * std::ldexp(1.0, -k));
+      // build quanta: the coarsest power-of-two multiple of the fixed-point quantum at or
+      // below 1/8192 of the largest cell edge, coarsened until a region coordinate (half a
+      // brick plus one cell) stays within int16
+      const double fq = 1.0 / s.fx_q, hmax = std::max(s.hc[0], std::max(s.hc[1], s.hc[2]));
+      const int bdmax = std::max(sp.bdim[0], std::max(sp.bdim[1], sp.bdim[2]));
+      int sh = 0;
+      while (fq * std::ldexp(1.0, sh + 1) <= hmax / 8192.0) ++sh;
+      while ((0.5 * bdmax + 1.0) * hmax / (fq * std::ldexp(1.0, sh)) > 30000.0) ++sh;
+      const double bq = fq * std::ldexp(1.0, sh);
+      ARG_CHECK(sh < 31 && 2.0 * hmax / bq < 32000.0, "system %d: no integer list quanta for cells of %.3g", si, hmax);
+      s.bq_sh = sh;
+      s.bq_inv = 1.0 / bq;
+      s.bq_m = (float)bq;
+      // a listed pair: squared quantised distance below T, which covers rc_list and the
+      // rounding of both atoms (each coordinate off by at most half a quantum)
+      const double rcq = std::sqrt(s.rc_list2) / bq + std::sqrt(3.0);
+      s.bq_T = (int)std::ceil(rcq * rcq) + 2;
+      s.bq_W = (int)std::ceil(rcq) + 1;
+      // the column prune tests the rounded atom position against exact cell boxes
+      const double rcp = std::sqrt(s.rc_list2) + 3.0 * std::sqrt(3.0) * bq;
+      s.bq_rcp2 = (float)(rcp * rcp * (1.0 + 1e-5));
     }
     // bricks and brick-major cell numbering
     const int nbx[3] = {(s.nc[0] + sp.bdim[0] - 1) / sp.bdim[0], (s.nc[1] + sp.bdim[1] - 1) / sp.bdim[1],

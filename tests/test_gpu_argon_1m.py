@@ -7,7 +7,8 @@ Checks (the reference's LJ is forces.py:119-142, its cell list forces.py:279-321
   - the positions are the committed configuration (fingerprint from
     tests/golden/make_c2_pairs.py);
   - list entries lie between twice the oracle's pair counts within rc_list and within
-    rc_list (1 + 1e-5), both counted by the oracle's cell_list_pairs on this configuration;
+    rc_list + 2 sqrt(3) h / 4096 (h the cell edge: the margin of the build's integer test),
+    both counted by the oracle's cell_list_pairs on this configuration;
   - the float64 forces of 200 sampled atoms equal the oracle's LennardJones evaluated on
     a periodic sub-system that holds every atom within the list radius of them (1e-10);
   - mixed against float64: RMS relative force error <= 1e-5 over all atoms, potential

@@ -1,9 +1,11 @@
 """Completeness of the brick engine's Verlet lists (csrc/brick.hip, k_b_build): right after
 set-up the number of list entries (a full list: every pair counted from both atoms) lies
 between twice the brute-force count of pairs within the list radius rc_list and twice the
-count within rc_list (1 + 1e-5) (the build's float32 test carries a 1e-5 margin on
-rc_list^2). The build scans z-sorted column windows, so a window that cut off a pair would
-show here as a count below the lower bound."""
+count within rc_list + 2 sqrt(3) h / 4096, h the cell edge: the build tests integer
+coordinates on a quantum of at most h / 4096 against a threshold that covers rc_list plus
+the rounding of both atoms (at most sqrt(3) quanta each way). The build scans z-sorted
+column windows, so a window that cut off a pair would show here as a count below the lower
+bound."""
 import numpy as np
 import pytest
 
@@ -17,6 +19,8 @@ from devinproj_amd import batch  # noqa: E402
 
 def _pair_counts(s, rc_list):
     L = np.asarray(s.box, float)
+    h = float(np.max(L / np.floor(L / rc_list)))  # the engine's cell edge (cells of at least rc_list)
+    rc_hi = rc_list + 2.0 * np.sqrt(3.0) * h / 4096.0
     n = s.n
     lo = hi = 0
     for i in range(0, n, 512):
@@ -26,7 +30,7 @@ def _pair_counts(s, rc_list):
         idx = np.arange(i, min(i + 512, n))
         r2[np.arange(len(idx)), idx] = np.inf
         lo += int(np.sum(r2 < rc_list ** 2))
-        hi += int(np.sum(r2 < rc_list ** 2 * (1 + 1e-5)))
+        hi += int(np.sum(r2 < rc_hi ** 2 * (1 + 1e-9)))
     return lo, hi
 
 
