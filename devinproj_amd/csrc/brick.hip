@@ -1273,12 +1273,14 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       const int ng = (nn[a] + kE - 1) / kE;
       P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
       unsigned ni = 0;  // pairs inside the cutoff (one species)
-      // groups of kE neighbours; the next group's indices load while this one computes
-      uint4 cur = make_uint4(0u, 0u, 0u, 0u);
+      // groups of kE neighbours; the groups two ahead load while this one computes (the
+      // step waits on its global loads, so two in flight per lane beat one)
+      uint4 cur = make_uint4(0u, 0u, 0u, 0u), nx1 = cur;
       if (sub < ng) cur = nbr[(size_t)sub * stride + a];
+      if (sub + gstep < ng) nx1 = nbr[(size_t)(sub + gstep) * stride + a];
       for (int g = sub; g < ng; g += gstep) {
-        uint4 nxt = cur;
-        if (g + gstep < ng) nxt = nbr[(size_t)(g + gstep) * stride + a];
+        uint4 nx2 = nx1;
+        if (g + 2 * gstep < ng) nx2 = nbr[(size_t)(g + 2 * gstep) * stride + a];
         // half a group at a time: its slot reads are in flight together
         constexpr int kC = kE / 2;
 #pragma unroll
@@ -1294,7 +1296,8 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
           fx = fy = fz = u = n = cx = cy = cz = cu = 0;
         }
-        cur = nxt;
+        cur = nx1;
+        nx1 = nx2;
       }
       Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
       Nin += (double)ni;
