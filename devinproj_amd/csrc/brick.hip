@@ -234,21 +234,29 @@ __device__ void stage_region(const BBrick& b, const BSys& s, const BRegion* __re
 #pragma unroll
     for (int d = 0; d < 3; ++d)  // region cell 0 starts 1 + hd/2 cells below the brick centre
       cen[d] = (rr[d] - 0.5 - 0.5 * b.hd[d]) * (s.L[d] / s.nc[d]);
-    for (int q = q0 + sub; q < q1; q += G) {
-      const double4 x = pos[base + q];
-      const double xs[3] = {x.x, x.y, x.z};
-      double X[3];
+    // two slots per thread and step, their loads in flight together
+    for (int q = q0 + sub; q < q1; q += 2 * G) {
+      double4 xx[2];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double e = xs[d] - b.org[d];
-        X[d] = fma(-s.L[d], rint((e - cen[d]) * s.inv[d]), e);
-      }
-      typename P4<P>::T v;
-      v.x = (P)(X[0] * scale);
-      v.y = (P)(X[1] * scale);
-      v.z = (P)(X[2] * scale);
-      v.w = (P)x.w;
-      st_slot<PK3>(lds, q, v);
+      for (int u = 0; u < 2; ++u)
+        if (q + u * G < q1) xx[u] = pos[base + q + u * G];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (q + u * G < q1) {
+          const double xs[3] = {xx[u].x, xx[u].y, xx[u].z};
+          double X[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double e = xs[d] - b.org[d];
+            X[d] = fma(-s.L[d], rint((e - cen[d]) * s.inv[d]), e);
+          }
+          typename P4<P>::T v;
+          v.x = (P)(X[0] * scale);
+          v.y = (P)(X[1] * scale);
+          v.z = (P)(X[2] * scale);
+          v.w = (P)xx[u].w;
+          st_slot<PK3>(lds, q + u * G, v);
+        }
     }
   }
 }
@@ -332,14 +340,22 @@ __device__ void stage_region_fix(const BBrick& b, const BSys& s, const BRegion* 
     int B[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) B[d] = (int)rint((c3[d] + sh[d] * s.L[d] - b.org[d]) * s.fx_q);
-    for (int q = q0 + sub; q < q1; q += G) {
-      const int4 o = posf[base + q];
-      float4 v;
-      v.x = (float)(B[0] + o.x) * scale;
-      v.y = (float)(B[1] + o.y) * scale;
-      v.z = (float)(B[2] + o.z) * scale;
-      v.w = (float)o.w;
-      st_slot<PK3>(lds, q, v);
+    // four slots per thread and step, their loads in flight together
+    for (int q = q0 + sub; q < q1; q += 4 * G) {
+      int4 o[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q + u * G < q1) o[u] = posf[base + q + u * G];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q + u * G < q1) {
+          float4 v;
+          v.x = (float)(B[0] + o[u].x) * scale;
+          v.y = (float)(B[1] + o[u].y) * scale;
+          v.z = (float)(B[2] + o[u].z) * scale;
+          v.w = (float)o[u].w;
+          st_slot<PK3>(lds, q + u * G, v);
+        }
     }
   }
 }
@@ -372,9 +388,16 @@ __device__ void stage_region_q(const BBrick& b, const BSys& s, const BRegion* __
       int B[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) B[d] = (int)rint((c3[d] + shv[d] * s.L[d] - b.org[d]) * s.fx_q);
-      for (int q = q0 + sub; q < q1; q += G) {
-        const int4 o = posf[base + q];
-        lds[q] = pack_q((B[0] + o.x + half) >> sh, (B[1] + o.y + half) >> sh, (B[2] + o.z + half) >> sh);
+      for (int q = q0 + sub; q < q1; q += 2 * G) {  // two loads in flight per thread
+        int4 o[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (q + u * G < q1) o[u] = posf[base + q + u * G];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (q + u * G < q1)
+            lds[q + u * G] = pack_q((B[0] + o[u].x + half) >> sh, (B[1] + o[u].y + half) >> sh,
+                                    (B[2] + o[u].z + half) >> sh);
       }
     } else {
       const int rr[3] = {r / (b.rd[2] * b.rd[1]), (r / b.rd[2]) % b.rd[1], r % b.rd[2]};
@@ -382,16 +405,23 @@ __device__ void stage_region_q(const BBrick& b, const BSys& s, const BRegion* __
 #pragma unroll
       for (int d = 0; d < 3; ++d)  // region cell 0 starts 1 + hd/2 cells below the brick centre
         cen[d] = (rr[d] - 0.5 - 0.5 * b.hd[d]) * (s.L[d] / s.nc[d]);
-      for (int q = q0 + sub; q < q1; q += G) {
-        const double4 x = pos[base + q];
-        const double xs[3] = {x.x, x.y, x.z};
-        int X[3];
+      for (int q = q0 + sub; q < q1; q += 2 * G) {  // two loads in flight per thread
+        double4 xx[2];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double e = xs[d] - b.org[d];
-          X[d] = (int)rint(fma(-s.L[d], rint((e - cen[d]) * s.inv[d]), e) * s.bq_inv);
-        }
-        lds[q] = pack_q(X[0], X[1], X[2]);
+        for (int u = 0; u < 2; ++u)
+          if (q + u * G < q1) xx[u] = pos[base + q + u * G];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (q + u * G < q1) {
+            const double xs[3] = {xx[u].x, xx[u].y, xx[u].z};
+            int X[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const double e = xs[d] - b.org[d];
+              X[d] = (int)rint(fma(-s.L[d], rint((e - cen[d]) * s.inv[d]), e) * s.bq_inv);
+            }
+            lds[q + u * G] = pack_q(X[0], X[1], X[2]);
+          }
       }
     }
   }
@@ -1050,8 +1080,9 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
 }
 
 // Verlet lists of the bricks of the systems the rebuild launch handed over.
+// eight waves per SIMD (64 VGPRs): four 512-thread bricks share a CU
 template <bool W12>
-__global__ void __launch_bounds__(kBT) k_b_build(RB p) {
+__global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(8))) k_b_build(RB p) {
   if (p.bflags[p.S] == 0 || p.ints[2] != 0) return;
   const BBrick b = p.bricks[blockIdx.x];
   if (!p.bflags[b.sys]) return;
