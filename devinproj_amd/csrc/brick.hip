@@ -1165,10 +1165,12 @@ __device__ __forceinline__ void lj_engine(P dx, P dy, P dz, P r2, P e2, P m24, P
   }
 }
 
+// (cp: the system's DSF constants copied into registers before the pair loop; qt: the
+// species charges staged in the LDS, so no pair reads global memory)
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN>
-__device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P qi, const typename P4<P>::T& xi,
-                                       const typename P4<P>::T& xj, P& fx, P& fy, P& fz, P& u, P& n,
-                                       unsigned& ni, P& cx, P& cy, P& cz, P& cu) {
+__device__ __forceinline__ void b_pair(const CoulParams& cp, const P* ljt, const P* qt, int row, P qi,
+                                       const typename P4<P>::T& xi, const typename P4<P>::T& xj, P& fx, P& fy,
+                                       P& fz, P& u, P& n, unsigned& ni, P& cx, P& cy, P& cz, P& cu) {
   const P dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
   const int tj = ONE_SP ? 0 : (int)xj.w;
   // list membership is decided at the build, so r^2 can use fused multiply-adds here
@@ -1180,10 +1182,10 @@ __device__ __forceinline__ void b_pair(const BSys& s, const P* ljt, int row, P q
   }
   if constexpr (COUL_ON) {
     if constexpr (sizeof(P) == 8)
-      coul_pair64(s.coul, dx, dy, dz, r2, qi * s.charge[tj], cx, cy, cz, cu);
+      coul_pair64(cp, dx, dy, dz, r2, qi * qt[tj], cx, cy, cz, cu);
     else
-      coul_pair32((float)s.coul.alpha, (float)s.coul.rc, (float)s.coul.c, (float)s.coul.erc_rc, (float)s.coul.fshift,
-                  s.coul.shifted, dx, dy, dz, r2, qi * (float)s.charge[tj], cx, cy, cz, cu);
+      coul_pair32((float)cp.alpha, (float)cp.rc, (float)cp.c, (float)cp.erc_rc, (float)cp.fshift, cp.shifted, dx, dy,
+                  dz, r2, qi * qt[tj], cx, cy, cz, cu);
   }
 }
 
@@ -1223,6 +1225,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
   __shared__ int hself[kMaxHomeCells];
   __shared__ int wsum[kBT / 64];
   __shared__ P ljt[4 * 64];
+  __shared__ P qtab[64];
   __shared__ double red[kRecP * 8];
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T4* lds = reinterpret_cast<T4*>(lds_raw);
@@ -1247,6 +1250,11 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       ljt[128 + k] = (P)s.lj_sig2[k];
       ljt[192 + k] = on ? (P)(UNIT_SIG ? s.lj_rc2[k] / s.lj_sig2[k] : s.lj_rc2[k]) : (P)-1;
     }
+  }
+  CoulParams cpar = {};
+  if constexpr (COUL_ON) {
+    cpar = s.coul;
+    for (int k = threadIdx.x; k < s.ns; k += blockDim.x) qtab[k] = (P)s.charge[k];
   }
   const int total = region_setup(b, reg, start, rs, wsum, hself);
   // uniform: an overflowing region (or a failed rebuild) skips the work; the host reports it
@@ -1321,7 +1329,8 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           for (int k = 0; k < kC; ++k) xj[k] = ld_slot<PK3>(lds, list_entry<W12>(cur, h2 + k));
 #pragma unroll
           for (int k = 0; k < kC; ++k)
-            b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(s, ljt, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, cx, cy, cz, cu);
+            b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(cpar, ljt, qtab, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, cx, cy,
+                                                  cz, cu);
         }
         if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every group
           Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
