@@ -1200,7 +1200,7 @@ __device__ __forceinline__ void b_pair(const CoulParams& cp, const P* ljt, const
 // do); the other forms keep the compiler's own allocation (bounding them spills).
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 constexpr int fused_waves() {
-  return sizeof(P) == 4 && LJ_ON && !COUL_ON && ONE_SP ? 6 : 1;
+  return sizeof(P) == 4 && LJ_ON && !COUL_ON && ONE_SP ? 6 : (sizeof(P) == 4 && COUL_ON ? 4 : 1);
 }
 
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN, bool W12>
@@ -1289,16 +1289,28 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
     const double h_cur = 0.5 * wc * s.dt, h_next = 0.5 * wn * s.dt, dt_next = wn * s.dt;
     const bool do_drift = (int)blockIdx.x < drift_from;
     const int a0 = start[b.cell0], a1 = start[b.cell0 + b.nhome];
-    for (int pass = a0; pass < a1; pass += blockDim.x) {  // uniform trip count
-      // a pass with at most half as many atoms as threads (the last pass of a brick with
-      // more atoms than threads, or a small brick) gives each atom L lanes (a power of two
-      // up to kSplit), which take its neighbour groups in turn and then sum their partial
-      // forces in a fixed shuffle order, so that the pass costs about 1/L of a full one
+    const int nwave = (int)blockDim.x >> 6;
+    for (int pass = a0; pass < a1;) {  // uniform trip count
+      // a pass whose atoms fit the block with L > 1 lanes each (the last pass of a brick
+      // with more atoms than threads, or a small brick) gives each atom L lanes, up to
+      // kSplit: the largest L for which 64 / L atoms per wave still cover the pass. The
+      // lanes of an atom are adjacent in one wave (lanes past (64 / L) L idle) and take its
+      // neighbour groups in turn, then sum their partial forces into the first of them in
+      // a fixed tree order, so that the pass costs about 1/L of a full one
+      // (LJ alone: L a power of two, so that (64 / L) L = 64; the general packing made the
+      // argon step 4 % slower)
       int L = 1;
-      while (2 * L <= kSplit && (a1 - pass) * 2 * L <= (int)blockDim.x) L *= 2;
+      if constexpr (COUL_ON) {
+        while (L < kSplit && nwave * (64 / (L + 1)) >= a1 - pass) ++L;
+      } else {
+        while (2 * L <= kSplit && (a1 - pass) * 2 * L <= (int)blockDim.x) L *= 2;
+      }
       const bool split = L > 1;
-      const int a = pass + (int)threadIdx.x / L;
-      const int sub = (int)threadIdx.x % L, gstep = L;
+      const int apw = 64 / L, lane = (int)threadIdx.x & 63;
+      const int a = !COUL_ON ? pass + (int)threadIdx.x / L
+                             : (lane < apw * L ? pass + ((int)threadIdx.x >> 6) * apw + lane / L : a1);
+      const int sub = lane % L, gstep = L;
+      pass += COUL_ON ? nwave * apw : (int)blockDim.x;
       double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
       double Fx = 0, Fy = 0, Fz = 0, U = 0, Nin = 0, Cx = 0, Cy = 0, Cz = 0, CU = 0;
       int ti = 0;
@@ -1312,16 +1324,19 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       const int ng = (nn[a] + kE - 1) / kE;
       P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
       unsigned ni = 0;  // pairs inside the cutoff (one species)
-      // groups of kE neighbours; the groups two ahead load while this one computes (the
-      // step waits on its global loads, so two in flight per lane beat one)
+      // groups of kE neighbours; the groups kAhead ahead load while this one computes (the
+      // LJ step waits on its global loads, so two in flight per lane beat one; a group of
+      // Coulomb pairs is long enough to cover the next one's load, and the registers of a
+      // second group would spill)
+      constexpr int kAhead = COUL_ON ? 1 : 2;
       uint4 cur = make_uint4(0u, 0u, 0u, 0u), nx1 = cur;
       if (sub < ng) cur = nbr[(size_t)sub * stride + a];
-      if (sub + gstep < ng) nx1 = nbr[(size_t)(sub + gstep) * stride + a];
+      if (kAhead == 2 && sub + gstep < ng) nx1 = nbr[(size_t)(sub + gstep) * stride + a];
       for (int g = sub; g < ng; g += gstep) {
         uint4 nx2 = nx1;
-        if (g + 2 * gstep < ng) nx2 = nbr[(size_t)(g + 2 * gstep) * stride + a];
+        if (g + kAhead * gstep < ng) nx2 = nbr[(size_t)(g + kAhead * gstep) * stride + a];
         // half a group at a time: its slot reads are in flight together
-        constexpr int kC = kE / 2;
+        constexpr int kC = COUL_ON ? 2 : kE / 2;
 #pragma unroll
         for (int h2 = 0; h2 < kE; h2 += kC) {
           T4 xj[kC];
@@ -1336,28 +1351,45 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
           fx = fy = fz = u = n = cx = cy = cz = cu = 0;
         }
-        cur = nx1;
-        nx1 = nx2;
+        if constexpr (kAhead == 2) {
+          cur = nx1;
+          nx1 = nx2;
+        } else {
+          cur = nx2;
+        }
       }
       Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
       Nin += (double)ni;
       }
       if (split) {
+        // Coulomb: a tree toward the atom's first lane: at distance o, lanes whose index
+        // within the atom is a multiple of 2 o add the lane o above them when it belongs to
+        // the atom. LJ alone (L a power of two): a butterfly, every lane of the atom ends
+        // with the sum.
 #pragma unroll
         for (int o = 1; o < kSplit; o <<= 1) {
           if (o >= L) continue;  // L is the same for the whole block
-          Fx += __shfl_xor(Fx, o, 64);
-          Fy += __shfl_xor(Fy, o, 64);
-          Fz += __shfl_xor(Fz, o, 64);
+          const bool take = (sub & (2 * o - 1)) == 0 && sub + o < L;
+          auto down = [&](double& v) {
+            if constexpr (COUL_ON) {
+              const double t = __shfl_down(v, o, 64);
+              if (take) v += t;
+            } else {
+              v += __shfl_xor(v, o, 64);
+            }
+          };
+          down(Fx);
+          down(Fy);
+          down(Fz);
           if constexpr (EN) {
-            U += __shfl_xor(U, o, 64);
-            Nin += __shfl_xor(Nin, o, 64);
+            down(U);
+            down(Nin);
           }
           if constexpr (COUL_ON) {
-            Cx += __shfl_xor(Cx, o, 64);
-            Cy += __shfl_xor(Cy, o, 64);
-            Cz += __shfl_xor(Cz, o, 64);
-            if constexpr (EN) CU += __shfl_xor(CU, o, 64);
+            down(Cx);
+            down(Cy);
+            down(Cz);
+            if constexpr (EN) down(CU);
           }
         }
       }

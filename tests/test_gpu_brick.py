@@ -64,6 +64,34 @@ def test_brick_lj_plus_dsf_matches_oracle():
     assert np.max(np.abs(s.pos - so.pos)) <= 1e-10 * so.box[0]
 
 
+@pytest.mark.parametrize("precision,tol,ptol", [("fp64", 1e-9, 1e-10), ("mixed", 1e-6, 1e-8)])
+def test_brick_coulomb_lane_split_matches_oracle(precision, tol, ptol):
+    """With Coulomb, a short pass gives each ion L lanes for any L up to 8 (64 // L ions per
+    wave, the lanes' partial forces summed by a shuffle-down tree). One-cell bricks of about
+    27 ions (LJ at 1.5 sigma, DSF at 8 A: cells of about 9.6 A) at 64 to 512 threads take L
+    through 2, 4, 5-7 and 8; every launch shape follows the oracle."""
+    s0 = B.rocksalt(24, 24, 24, seed=5)
+    rng = np.random.default_rng(4)
+    s0.pos = (s0.pos + rng.normal(0, 0.2e-10, s0.pos.shape)) % s0.box
+    so = s0.copy()
+    sim = A.Simulation(so, [A.LennardJones(B.NACL_LJ, cutoff=1.5), A.DampedShiftedCoulomb(rcut=8e-10, alpha=0.25e10)],
+                       integrator=A.VelocityVerlet(), dt=1e-15)
+    tr = sim.run(steps=4, positions=False)
+    for threads in (64, 128, 192, 256, 320, 512):
+        s = s0.copy()
+        forces = [dp.LennardJones(B.NACL_LJ, cutoff=1.5, precision=precision),
+                  dp.DampedShiftedCoulomb(rcut=8e-10, alpha=0.25e10, precision=precision)]
+        eng = batch.BrickEngine([s], [forces], dp.VelocityVerlet(), 1e-15, precision=precision, bricks=[(1, 1, 1)],
+                                threads=threads)
+        eng.run(4)
+        eng.fetch()
+        assert np.allclose(eng.logs[0][:, 0], tr.potential, rtol=tol, atol=0), threads
+        assert np.allclose(eng.logs[0][:, 1], tr.kinetic, rtol=tol, atol=0), threads
+        # positions: float32 pair forces move the mixed trajectory by about 1e-17 m in 4 fs
+        assert np.max(np.abs(s.pos - so.pos)) <= ptol * so.box[0], threads
+        eng.close()
+
+
 def test_batched_systems_equal_single_runs_bitwise():
     systems = [B.lj_reduced(n, rho, seed=k) for k, (n, rho) in enumerate([(4000, 0.8), (6000, 0.75), (5000, 0.85)])]
     mk = lambda: [dp.LennardJones({"A": (1.0, 1.0)}, cutoff=2.5, skin=0.3)]
