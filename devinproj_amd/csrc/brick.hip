@@ -284,13 +284,15 @@ struct BState {
   int* perm[2];     // user index of each slot
   int* cell[2];     // cell id of each slot
   // mixed mode only (else null): the positions as fixed-point offsets from the corner of
-  // each atom's cell, species in w; posf[0] pairs with posA and posf[1] with posB (the
-  // position parity, not the state parity)
-  int4* posf[2];
+  // each atom's cell, fxw ints per atom: x, y, z and, with several species, the species;
+  // posf[0] pairs with posA and posf[1] with posB (the position parity, not the state
+  // parity)
+  int* posf[2];
   // mixed mode only (else null): the fixed-point positions at the last list build, in the
   // layout of posf and the state parity of ref (which mixed mode leaves unused)
-  int4* refq[2];
+  int* refq[2];
   const double4* ccorner;  // corner of every cell in the box frame (both modes)
+  int fxw;                 // ints per atom of posf and refq: 3 with one species, else 4
 };
 
 // Fixed-point copy of position x of an atom of the cell with corner cc: per axis the offset
@@ -305,6 +307,32 @@ __device__ __forceinline__ int4 fix_pos(const BSys& s, const double4& x, const d
     q[d] = (int)rint(e * s.fx_q);
   }
   return make_int4(q[0], q[1], q[2], (int)x.w);
+}
+
+// One atom's record of posf / refq: 12 bytes (x, y, z) with one species, else 16 (the
+// species in w). The 12-byte record is read and written as one dwordx3 access.
+struct __attribute__((aligned(4))) Fix3 {
+  int x, y, z;
+};
+template <bool W3>
+__device__ __forceinline__ int4 ld_fix(const int* __restrict__ p, int i) {
+  if constexpr (W3) {
+    const Fix3 v = *reinterpret_cast<const Fix3*>(p + 3 * (size_t)i);
+    return make_int4(v.x, v.y, v.z, 0);
+  } else {
+    return *reinterpret_cast<const int4*>(p + 4 * (size_t)i);
+  }
+}
+__device__ __forceinline__ int4 ld_fix(const int* __restrict__ p, int i, int w) {
+  return w == 3 ? ld_fix<true>(p, i) : ld_fix<false>(p, i);
+}
+__device__ __forceinline__ void st_fix(int* __restrict__ p, int i, int w, const int4& v) {
+  if (w == 3) {
+    const Fix3 t = {v.x, v.y, v.z};
+    *reinterpret_cast<Fix3*>(p + 3 * (size_t)i) = t;
+  } else {
+    *reinterpret_cast<int4*>(p + 4 * (size_t)i) = v;
+  }
 }
 
 // Mixed mode: whether an atom has moved more than half the skin since the last list
@@ -322,9 +350,9 @@ __device__ __forceinline__ bool moved_far_q(const BSys& s, const int4& f, const 
 // then one rounding to float and an exact power-of-two scale (`scale` = 2^-fx_k), which is
 // the value the float64 staging rounds to, up to one quantum. 16 bytes per atom read
 // instead of 32, and no float64 image arithmetic per atom.
-template <bool PK3>
+template <bool PK3, bool W3>
 __device__ void stage_region_fix(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
-                                 const int* __restrict__ start, const int* rs, const int4* __restrict__ posf,
+                                 const int* __restrict__ start, const int* rs, const int* __restrict__ posf,
                                  const double4* __restrict__ ccorner, float4* lds, float scale) {
   const int R = b.rd[0] * b.rd[1] * b.rd[2];
   const int G = max(1, (int)blockDim.x / R);  // threads per region cell
@@ -345,7 +373,7 @@ __device__ void stage_region_fix(const BBrick& b, const BSys& s, const BRegion* 
       int4 o[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (q + u * G < q1) o[u] = posf[base + q + u * G];
+        if (q + u * G < q1) o[u] = ld_fix<W3>(posf, base + q + u * G);
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (q + u * G < q1) {
@@ -353,7 +381,7 @@ __device__ void stage_region_fix(const BBrick& b, const BSys& s, const BRegion* 
           v.x = (float)(B[0] + o[u].x) * scale;
           v.y = (float)(B[1] + o[u].y) * scale;
           v.z = (float)(B[2] + o[u].z) * scale;
-          v.w = (float)o[u].w;
+          v.w = W3 ? 0.0f : (float)o[u].w;  // one species: species 0
           st_slot<PK3>(lds, q + u * G, v);
         }
     }
@@ -370,8 +398,9 @@ __device__ __forceinline__ uint2 pack_q(int x, int y, int z) {
 }
 template <bool FIX>
 __device__ void stage_region_q(const BBrick& b, const BSys& s, const BRegion* __restrict__ reg,
-                               const int* __restrict__ start, const int* rs, const int4* __restrict__ posf,
-                               const double4* __restrict__ pos, const double4* __restrict__ ccorner, uint2* lds) {
+                               const int* __restrict__ start, const int* rs, const int* __restrict__ posf,
+                               int fxw, const double4* __restrict__ pos, const double4* __restrict__ ccorner,
+                               uint2* lds) {
   const int R = b.rd[0] * b.rd[1] * b.rd[2];
   const int G = max(1, (int)blockDim.x / R);  // threads per region cell
   const int per = (int)blockDim.x / G;         // region cells per pass
@@ -392,7 +421,7 @@ __device__ void stage_region_q(const BBrick& b, const BSys& s, const BRegion* __
         int4 o[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-          if (q + u * G < q1) o[u] = posf[base + q + u * G];
+          if (q + u * G < q1) o[u] = ld_fix(posf, base + q + u * G, fxw);
 #pragma unroll
         for (int u = 0; u < 2; ++u)
           if (q + u * G < q1)
@@ -623,9 +652,10 @@ __device__ void build_brick(const RB& p, const BBrick& b, uint2* lds, int* rs, i
     return;
   }
   if (p.st.posf[0])
-    stage_region_q<true>(b, s, p.reg, p.start, rs, p.st.posf[p.par[b.sys]], nullptr, p.st.ccorner, lds);
+    stage_region_q<true>(b, s, p.reg, p.start, rs, p.st.posf[p.par[b.sys]], p.st.fxw, nullptr, p.st.ccorner, lds);
   else
-    stage_region_q<false>(b, s, p.reg, p.start, rs, nullptr, cur_buf(p.posA, p.posB, p.par, b.sys), nullptr, lds);
+    stage_region_q<false>(b, s, p.reg, p.start, rs, nullptr, 0, cur_buf(p.posA, p.posB, p.par, b.sys), nullptr,
+                          lds);
   __syncthreads();
   const float h[3] = {(float)(s.L[0] / s.nc[0]), (float)(s.L[1] / s.nc[1]), (float)(s.L[2] / s.nc[2])};
   const float c0[3] = {-(1.0f + 0.5f * b.hd[0]) * h[0], -(1.0f + 0.5f * b.hd[1]) * h[1],
@@ -922,8 +952,8 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       (pc ? p.posA : p.posB)[t] = x;
       if (p.st.posf[0]) {
         const int4 f = fix_pos(p.sys[si], x, p.st.ccorner[c]);
-        p.st.posf[pc ^ 1][t] = f;
-        p.st.refq[sc ^ 1][t] = f;
+        st_fix(p.st.posf[pc ^ 1], t, p.st.fxw, f);
+        st_fix(p.st.refq[sc ^ 1], t, p.st.fxw, f);
       } else {
         p.st.ref[sc ^ 1][t] = x;
       }
@@ -974,8 +1004,8 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       (pc ? p.posA : p.posB)[t] = q.x;
       if (p.st.posf[0]) {
         const int4 f = fix_pos(p.sys[q.si], q.x, p.st.ccorner[c]);
-        p.st.posf[pc ^ 1][t] = f;
-        p.st.refq[sc ^ 1][t] = f;
+        st_fix(p.st.posf[pc ^ 1], t, p.st.fxw, f);
+        st_fix(p.st.refq[sc ^ 1], t, p.st.fxw, f);
       } else {
         p.st.ref[sc ^ 1][t] = q.x;
       }
@@ -1197,7 +1227,10 @@ __device__ __forceinline__ void b_pair(const CoulParams& cp, const P* ljt, const
 // frame snap_count[0] of the snapshot ring (frames of N positions then N velocities).
 // Waves per SIMD the compiler must leave room for: the float32 one-species LJ step fits
 // 80 VGPRs without spilling, so three 512-thread bricks share a CU (as their LDS regions
-// do); the other forms keep the compiler's own allocation (bounding them spills).
+// do); the float32 Coulomb steps are held to 128 VGPRs, four waves instead of three (the
+// two-species recorded form spills a few values, three scratch accesses per group of ten
+// pairs, and still runs 12 % faster; at 96 VGPRs it spills 77); the float64 forms keep the
+// compiler's own allocation.
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 constexpr int fused_waves() {
   return sizeof(P) == 4 && LJ_ON && !COUL_ON && ONE_SP ? 6 : (sizeof(P) == 4 && COUL_ON ? 4 : 1);
@@ -1274,7 +1307,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
   }
   if (ok) {
     if constexpr (sizeof(P) == 4)
-      stage_region_fix<PK3>(b, s, reg, start, rs, sta.posf[par[si]], sta.ccorner, lds, s.fx_s);
+      stage_region_fix<PK3, ONE_SP>(b, s, reg, start, rs, sta.posf[par[si]], sta.ccorner, lds, s.fx_s);
     else
       stage_region<P, PK3>(b, s, reg, start, rs, pos, lds, 1.0 / unit);
     if (threadIdx.x == 0) {  // far sentinel: the padding entries of the lists point here
@@ -1464,8 +1497,8 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
         pos_next[a] = x;
         if constexpr (sizeof(P) == 4) {
           const int4 f = fix_pos(s, x, sta.ccorner[cell[a]]);
-          sta.posf[par[si] ^ 1][a] = f;
-          moved_far |= moved_far_q(s, f, sta.refq[sp][a]);
+          st_fix(sta.posf[par[si] ^ 1], a, ONE_SP ? 3 : 4, f);
+          moved_far |= moved_far_q(s, f, ld_fix<ONE_SP>(sta.refq[sp], a));
         } else {
           const double4 r = ref[a];
           const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
@@ -1629,8 +1662,8 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
       pos_next[a] = x;
       if (sta.posf[0]) {
         const int4 f = fix_pos(s, x, sta.ccorner[sta.cell[sp][a]]);
-        sta.posf[par[si] ^ 1][a] = f;
-        moved_far = moved_far_q(s, f, sta.refq[sp][a]);
+        st_fix(sta.posf[par[si] ^ 1], a, sta.fxw, f);
+        moved_far = moved_far_q(s, f, ld_fix(sta.refq[sp], a, sta.fxw));
       } else {
         const double4 r = ref[a];
         const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
@@ -1955,8 +1988,8 @@ __device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restric
     D(5 * j + 2) = x.z;
     if (sta.posf[0]) {
       const int4 f = fix_pos(s, x, sta.ccorner[sta.cell[sp][t]]);
-      sta.posf[par[si] ^ 1][t] = f;
-      far |= moved_far_q(s, f, sta.refq[sp][t]);
+      st_fix(sta.posf[par[si] ^ 1], t, sta.fxw, f);
+      far |= moved_far_q(s, f, ld_fix(sta.refq[sp], t, sta.fxw));
     } else {
       const double4 r = ref[t];
       const double d0 = minimg_fast(x.x - r.x, s.L[0], s.inv[0]);
@@ -2777,6 +2810,7 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   m.any_lj = lj0;
   m.any_coul = co0;
   m.one_sp = maxns == 1;
+  m.sta.fxw = m.one_sp ? 3 : 4;  // the fused launch's ONE_SP form reads 12-byte records
   int rc = build_tables(m);
   if (rc) return rc;
   HIP_TRY(hipMalloc(&m.count, (m.ncells + 1) * sizeof(int)));
