@@ -278,7 +278,7 @@ constexpr int kBarSpin = 1 << 20;  // bounded barrier wait (0.1 s or more): a st
 // Per-atom engine state except positions, double-buffered like the positions: system s
 // uses buffer spar[s]; a rebuild gathers into the other buffer and flips spar[s].
 struct BState {
-  double4* vel[2];
+  double* vel[2];  // velocities, 3 doubles per atom (ld_vel / st_vel)
   double4* F[2];
   double4* ref[2];  // positions at the last list build
   int* perm[2];     // user index of each slot
@@ -294,6 +294,20 @@ struct BState {
   const double4* ccorner;  // corner of every cell in the box frame (both modes)
   int fxw;                 // ints per atom of posf and refq: 3 with one species, else 4
 };
+
+// Velocities are kept as 24-byte records (x, y, z), not padded double4: 16 bytes less per
+// atom and step for the fused launch, which reads and writes every velocity.
+struct __attribute__((aligned(8))) Vel3 {
+  double x, y, z;
+};
+__device__ __forceinline__ double4 ld_vel(const double* __restrict__ v, int i) {
+  const Vel3 t = *reinterpret_cast<const Vel3*>(v + 3 * (size_t)i);
+  return make_double4(t.x, t.y, t.z, 0.0);
+}
+__device__ __forceinline__ void st_vel(double* __restrict__ v, int i, const double4& u) {
+  const Vel3 t = {u.x, u.y, u.z};
+  *reinterpret_cast<Vel3*>(v + 3 * (size_t)i) = t;
+}
 
 // Fixed-point copy of position x of an atom of the cell with corner cc: per axis the offset
 // from the corner at the periodic image nearest the cell's centre, in quanta of
@@ -957,7 +971,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       } else {
         p.st.ref[sc ^ 1][t] = x;
       }
-      p.st.vel[sc ^ 1][t] = p.st.vel[sc][v];
+      st_vel(p.st.vel[sc ^ 1], t, ld_vel(p.st.vel[sc], v));
       const int u = p.st.perm[sc][v];
       p.st.perm[sc ^ 1][t] = u;
       p.st.cell[sc ^ 1][t] = c;
@@ -991,7 +1005,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
             q.v = atoms_of(cm)[hl];
             const int pc = rb_par(p, q.si), sc = p.spar[q.si];
             q.x = (pc ? p.posB : p.posA)[q.v];
-            q.vv = p.st.vel[sc][q.v];
+            q.vv = ld_vel(p.st.vel[sc], q.v);
             q.u = p.st.perm[sc][q.v];
           }
         }
@@ -1009,7 +1023,7 @@ __global__ void __launch_bounds__(kRB) k_b_rebuild(RB p) {
       } else {
         p.st.ref[sc ^ 1][t] = q.x;
       }
-      p.st.vel[sc ^ 1][t] = q.vv;
+      st_vel(p.st.vel[sc ^ 1], t, q.vv);
       p.st.perm[sc ^ 1][t] = q.u;
       p.st.cell[sc ^ 1][t] = c;
       if (p.islot) p.islot[q.u] = t;
@@ -1266,7 +1280,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
   const double4* __restrict__ pos = par[si] ? posB : posA;
   double4* __restrict__ pos_next = par[si] ? posA : posB;
   const int sp = spar[si];
-  double4* __restrict__ vel = sta.vel[sp];
+  double* __restrict__ vel = sta.vel[sp];
   double4* __restrict__ Fst = sta.F[sp];
   const double4* __restrict__ ref = sta.ref[sp];
   const int* __restrict__ cell = sta.cell[sp];
@@ -1448,7 +1462,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       }
       const double m = s.mass[ti];
       const double ax = Fx / m, ay = Fy / m, az = Fz / m;
-      double4 v = vel[a];
+      double4 v = ld_vel(vel, a);
       {
 #pragma clang fp contract(off)
         v.x = v.x + h_cur * ax;
@@ -1507,7 +1521,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           moved_far |= !(r2_np(d0, d1, d2) < s.half_skin2);
         }
       }
-      vel[a] = v;
+      st_vel(vel, a, v);
       }
       if constexpr (EN) {
         block_sum<kRecP>(acc, red);
@@ -1599,14 +1613,14 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
     const double4* pos = par[si] ? posB : posA;
     double4* pos_next = par[si] ? posA : posB;
     const int sp = spar[si];
-    double4* vel = sta.vel[sp];
+    double* vel = sta.vel[sp];
     const double4* ref = sta.ref[sp];
     const double4 x0 = pos[a];
     const int ti = (int)x0.w;
     const double m = s.mass[ti];
     const double4 f = sta.F[sp][a];
     const double h = 0.5 * wn * s.dt, dtn = wn * s.dt;
-    double4 v = vel[a], x = x0;
+    double4 v = ld_vel(vel, a), x = x0;
     double n0 = 0, n1 = 0, n2 = 0;
     if (ig.mode == kLangevin || ig.mode == kBrownian) noise3(ig.seed, ndrift[si], (unsigned)sta.perm[sp][a], n0, n1, n2);
     {
@@ -1654,7 +1668,7 @@ __global__ void k_b_drift(const BSys* __restrict__ sys, const int* __restrict__ 
         }
       }
     }
-    vel[a] = v;
+    st_vel(vel, a, v);
     if (drift) {
       x.x = np_mod2(x.x, s.L[0]);
       x.y = np_mod2(x.y, s.L[1]);
@@ -1759,7 +1773,7 @@ __global__ void k_b_scatter_user(int natoms, int which, const double4* __restric
   if (t >= natoms) return;
   const int si = sysid[t], sp = spar[si];
   const int u = sta.perm[sp][t];
-  const double4 v = which == 0 ? (par[si] ? posB : posA)[t] : (which == 1 ? sta.vel[sp] : sta.F[sp])[t];
+  const double4 v = which == 0 ? (par[si] ? posB : posA)[t] : (which == 1 ? ld_vel(sta.vel[sp], t) : sta.F[sp][t]);
   out[3 * (size_t)u] = v.x;
   out[3 * (size_t)u + 1] = v.y;
   out[3 * (size_t)u + 2] = v.z;
@@ -1905,7 +1919,7 @@ __device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restric
   const double4* old = par[si] ? posB : posA;
   double4* nw = par[si] ? posA : posB;
   const int sp = spar[si];
-  double4* vel = sta.vel[sp];
+  double* vel = sta.vel[sp];
   const double4* ref = sta.ref[sp];
   const int k0 = c.coff[cl], K = c.coff[cl + 1] - k0, a0 = c.aoff[cl], na = c.aoff[cl + 1] - a0;
   const int* la = c.la + k0;
@@ -1974,7 +1988,7 @@ __device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restric
   bool far = false;
   for (int j = 0; j < na; ++j) {
     const int t = (int)D(5 * j + 4);
-    double4 x = nw[t], v = vel[t];
+    double4 x = nw[t], v = ld_vel(vel, t);
     x.x = np_mod2(x.x + D(5 * j), s.L[0]);
     x.y = np_mod2(x.y + D(5 * j + 1), s.L[1]);
     x.z = np_mod2(x.z + D(5 * j + 2), s.L[2]);
@@ -1982,7 +1996,7 @@ __device__ double shake_cluster(const BSys* __restrict__ sys, double4* __restric
     v.y = v.y + D(5 * j + 1) / s.dt;
     v.z = v.z + D(5 * j + 2) / s.dt;
     nw[t] = x;
-    vel[t] = v;
+    st_vel(vel, t, v);
     D(5 * j) = x.x;  // the displacement is applied: keep the final position for the check below
     D(5 * j + 1) = x.y;
     D(5 * j + 2) = x.z;
@@ -2045,7 +2059,7 @@ __global__ void __launch_bounds__(64) k_b_rattle(const BSys* __restrict__ sys, c
   const int si = c.csys[cl];
   const BSys& s = sys[si];
   const double4* pos = par[si] ? posB : posA;
-  double4* vel = sta.vel[spar[si]];
+  double* vel = sta.vel[spar[si]];
   const int k0 = c.coff[cl], K = c.coff[cl + 1] - k0, a0 = c.aoff[cl], na = c.aoff[cl + 1] - a0;
   const int* la = c.la + k0;
   const int* lb = c.lb + k0;
@@ -2059,7 +2073,7 @@ __global__ void __launch_bounds__(64) k_b_rattle(const BSys* __restrict__ sys, c
   }
   for (int k = 0; k < K; ++k) {
     const int ta = (int)D(5 * la[k] + 4), tb = (int)D(5 * lb[k] + 4);
-    const double4 pa = pos[ta], pb = pos[tb], va = vel[ta], vb = vel[tb];
+    const double4 pa = pos[ta], pb = pos[tb], va = ld_vel(vel, ta), vb = ld_vel(vel, tb);
     const double rx = mi_np(pa.x - pb.x, s.L[0]), ry = mi_np(pa.y - pb.y, s.L[1]), rz = mi_np(pa.z - pb.z, s.L[2]);
     W(8 * k) = rx;
     W(8 * k + 1) = ry;
@@ -2079,12 +2093,12 @@ __global__ void __launch_bounds__(64) k_b_rattle(const BSys* __restrict__ sys, c
   double acc[kRecP] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int j = 0; j < na; ++j) {
     const int t = (int)D(5 * j + 4);
-    const double4 v = vel[t];
+    const double4 v = ld_vel(vel, t);
     double4 u = v;
     u.x = v.x + D(5 * j);
     u.y = v.y + D(5 * j + 1);
     u.z = v.z + D(5 * j + 2);
-    vel[t] = u;
+    st_vel(vel, t, u);
     if (rec) {
       const double4 x = pos[t];
       const double m = s.mass[(int)x.w];
@@ -2720,8 +2734,8 @@ int accu_bmd_create(accu_bmd** out, int nsys, const int* natoms, const double* p
     perm[i] = (int)i;
   }
   HIP_TRY(hipMemcpy(m->pos, h4.data(), d4, hipMemcpyHostToDevice));
-  for (long long i = 0; i < N; ++i) h4[i] = make_double4(vel_host[3 * i], vel_host[3 * i + 1], vel_host[3 * i + 2], 0);
-  HIP_TRY(hipMemcpy(m->sta.vel[0], h4.data(), d4, hipMemcpyHostToDevice));
+  // velocities: 3 doubles per atom, slots in user order until the first sort
+  HIP_TRY(hipMemcpy(m->sta.vel[0], vel_host, (size_t)N * 3 * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(m->sta.F[0], 0, d4));
   HIP_TRY(hipMemcpy(m->sta.perm[0], perm.data(), i4, hipMemcpyHostToDevice));
   HIP_TRY(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
