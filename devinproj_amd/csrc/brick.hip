@@ -1239,15 +1239,16 @@ __device__ __forceinline__ void b_pair(const CoulParams& cp, const P* ljt, const
 // momentum and angular momentum partials (the pair energies are computed only then) and,
 // when snap is set, the positions and velocities of the recorded state in user order into
 // frame snap_count[0] of the snapshot ring (frames of N positions then N velocities).
-// Waves per SIMD the compiler must leave room for: the float32 one-species LJ step fits
-// 80 VGPRs without spilling, so three 512-thread bricks share a CU (as their LDS regions
-// do); the float32 Coulomb steps are held to 128 VGPRs, four waves instead of three (the
-// two-species recorded form spills a few values, three scratch accesses per group of ten
-// pairs, and still runs 12 % faster; at 96 VGPRs it spills 77); the float64 forms keep the
-// compiler's own allocation.
+// Waves per SIMD the compiler must leave room for: the float32 one-species LJ step and the
+// float32 Coulomb steps that merge their sums (several species, or no LJ) fit 80 VGPRs
+// without spilling, so three 512-thread bricks share a CU (as their LDS regions do); the
+// one-species LJ + Coulomb float32 step is held to 128 VGPRs (it spills at 80); the
+// float64 forms keep the compiler's own allocation (held to 128 they spill 36-60 values).
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP>
 constexpr int fused_waves() {
-  return sizeof(P) == 4 && LJ_ON && !COUL_ON && ONE_SP ? 6 : (sizeof(P) == 4 && COUL_ON ? 4 : 1);
+  constexpr bool merge = COUL_ON && (!LJ_ON || !ONE_SP);  // the kernel's MERGE
+  if constexpr (sizeof(P) == 4) return (LJ_ON && !COUL_ON && ONE_SP) || merge ? 6 : (COUL_ON ? 4 : 1);
+  return 1;
 }
 
 template <typename P, bool LJ_ON, bool COUL_ON, bool ONE_SP, bool EN, bool W12>
@@ -1265,6 +1266,10 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
   typedef typename P4<P>::T T4;
   // one species and no Coulomb: positions are staged in units of sigma (x = 1/r2)
   constexpr bool UNIT_SIG = LJ_ON && ONE_SP && !COUL_ON;
+  // Coulomb pair terms go into the LJ sums (the Coulomb constant folded into q_i) unless
+  // the one-species LJ factors are applied per atom after the loop: four float and four
+  // double accumulators fewer
+  constexpr bool MERGE = COUL_ON && (!LJ_ON || !ONE_SP);
   constexpr bool PK3 = UNIT_SIG && sizeof(P) == 4;  // 12-byte float32 slots (no species word)
   const BBrick b = bricks[blockIdx.x];
   const BSys& s = sys[b.sys];
@@ -1366,7 +1371,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
       const T4 xi = ld_slot<PK3>(lds, self);
       ti = ONE_SP ? 0 : (int)xi.w;
       const int row = ti * s.ns;
-      const P qi = (P)s.charge[ti];
+      const P qi = MERGE ? (P)(s.charge[ti] * s.coul.k) : (P)s.charge[ti];
       constexpr int kE = W12 ? 10 : 8;  // entries per 16-byte group of the list
       const int ng = (nn[a] + kE - 1) / kE;
       P fx = 0, fy = 0, fz = 0, u = 0, n = 0, cx = 0, cy = 0, cz = 0, cu = 0;
@@ -1391,8 +1396,12 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           for (int k = 0; k < kC; ++k) xj[k] = ld_slot<PK3>(lds, list_entry<W12>(cur, h2 + k));
 #pragma unroll
           for (int k = 0; k < kC; ++k)
-            b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(cpar, ljt, qtab, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, cx, cy,
-                                                  cz, cu);
+            if constexpr (MERGE)
+              b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(cpar, ljt, qtab, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, fx, fy,
+                                                    fz, u);
+            else
+              b_pair<P, LJ_ON, COUL_ON, ONE_SP, EN>(cpar, ljt, qtab, row, qi, xi, xj[k], fx, fy, fz, u, n, ni, cx, cy,
+                                                    cz, cu);
         }
         if constexpr (sizeof(P) == 4) {  // float32 pair terms, float64 sums every group
           Fx += fx; Fy += fy; Fz += fz; U += u; Nin += n; Cx += cx; Cy += cy; Cz += cz; CU += cu;
@@ -1432,7 +1441,7 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
             down(U);
             down(Nin);
           }
-          if constexpr (COUL_ON) {
+          if constexpr (COUL_ON && !MERGE) {
             down(Cx);
             down(Cy);
             down(Cz);
@@ -1452,7 +1461,10 @@ k_b_fused(const BSys* __restrict__ sys, const BBrick* __restrict__ bricks,
           U -= s.lj_shift * Nin;
         }
       }
-      if constexpr (COUL_ON) {
+      if constexpr (MERGE) {
+        const double q = s.charge[ti];
+        U -= s.coul.k * (q * q * s.coul.self);
+      } else if constexpr (COUL_ON) {
         const double q = s.charge[ti];
         CU -= q * q * s.coul.self;
         Fx += s.coul.k * Cx;
@@ -2906,8 +2918,9 @@ int accu_bmd_init(accu_bmd* h, double slack, int log_cap) {
   ARG_CHECK(m.lds_build() + 6144 <= lds_max && m.lds_fused() + 6144 <= lds_max && cap < 65536,
             "brick region of %d atoms does not fit the LDS (%zu bytes): use smaller bricks", cap, lds_max);
   // 12-bit entries need every slot index (the sentinel's included) below 4,096; the float64
-  // Coulomb step keeps 16-bit ones (ten of its pairs per group need more than 256 VGPRs and
-  // spill inside the pair loop, where eight spill a handful)
+  // Coulomb step keeps 16-bit ones (with its sums merged it no longer spills with ten pairs
+  // per group, but 12-bit lists did not make the NaCl fp64 step faster: 4,141 against
+  // 4,162 steps/s)
   m.w12 = cap <= 4095 && !(m.precision == ACCU_FP64 && m.any_coul);
   (void)hipFuncSetAttribute((const void*)m.build_kernel(),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds_build());
