@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # per dispatch, the factors being what the two counters read for a 1 GiB device copy
 # (bench.py --copy-probe 1024); the summary with its commands is
 # profiles/k_b_fused_lj_argon_1048576.md
-FUSED_TRAFFIC_BYTES = 4.163e8
+FUSED_TRAFFIC_BYTES = 3.749e8
 DT = 10e-15
 # brick tiles: the autotuner's choice on MI355X for each workload and precision
 # (python -m devinproj_amd.autotune writes devinproj_amd/data/tiles_mi355x.json); the
