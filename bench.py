@@ -42,6 +42,11 @@ HBM_PEAK_GBS = 8000.0
 # (bench.py --copy-probe 1024); the summary with its commands is
 # profiles/k_b_fused_lj_argon_1048576.md
 FUSED_TRAFFIC_BYTES = 3.749e8
+# VALU wave instructions per launch of the same kernel (rocprofv3 SQ_INSTS_VALU, summed over
+# the device, same profile) and the issue capacity they are set against: 256 CUs x 4 SIMDs,
+# one wave64 VALU instruction per 4 cycles each, at 2.4 GHz
+FUSED_VALU_INSTS = 4.86e7
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 DT = 10e-15
 # brick tiles: the autotuner's choice on MI355X for each workload and precision
 # (python -m devinproj_amd.autotune writes devinproj_amd/data/tiles_mi355x.json); the
@@ -516,7 +521,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": FUSED_TRAFFIC_BYTES,
                      "kernel": "k_b_fused<float,LJ>", "kernel_ms": mixed["kernel_ms"],
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     # the kernel's other limit: its measured VALU instructions over the issue
+                     # capacity of its measured launch time
+                     "valu_issue_frac": FUSED_VALU_INSTS / (1e-3 * mixed["kernel_ms"] * VALU_ISSUE_PER_S)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
